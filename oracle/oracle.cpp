@@ -1,0 +1,849 @@
+// CPU oracle (test infrastructure only).  See oracle.hpp for the restated
+// reference functions.  All arithmetic runs with the FPU in round-toward-minus-
+// infinity, upper bounds are computed as -RD(-x op y) like the reference, and
+// the upper bound of sqrt switches the rounding mode to upward around one
+// sqrtsd.  The value barrier `opq` keeps the compiler from folding the
+// negations (the build also uses -frounding-math).
+#include "oracle.hpp"
+#include <atomic>
+#include <cfenv>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- arithmetic
+static std::atomic<std::uint64_t> g_flops{0};
+static bool g_count_flops = false;
+thread_local std::uint64_t t_flops = 0;
+
+inline double opq(double x) { __asm__ volatile("" : "+x"(x)); return x; }
+inline void count(int k) { if(g_count_flops) t_flops += static_cast<std::uint64_t>(k); }
+
+inline double rd_add(double a, double b) { count(1); return opq(opq(a) + opq(b)); }
+inline double ru_add(double a, double b) { return -rd_add(-a, -b); }
+inline double rd_sub(double a, double b) { count(1); return opq(opq(a) - opq(b)); }
+inline double ru_sub(double a, double b) { return -rd_add(-a, b); }
+inline double rd_mul(double a, double b) { count(1); return opq(opq(a) * opq(b)); }
+inline double ru_mul(double a, double b) { return -rd_mul(-a, b); }
+inline double rd_div(double a, double b) { count(1); return opq(opq(a) / opq(b)); }
+inline double ru_div(double a, double b) { return -rd_div(-a, b); }
+inline double rd_sqrt(double a) { count(1); return opq(__builtin_sqrt(opq(a))); }
+inline double ru_sqrt(double a) {
+    count(1);
+    __asm__ volatile("" ::: "memory");
+    std::fesetround(FE_UPWARD);
+    __asm__ volatile("" ::: "memory");
+    double r = opq(__builtin_sqrt(opq(a)));
+    __asm__ volatile("" ::: "memory");
+    std::fesetround(FE_DOWNWARD);
+    __asm__ volatile("" ::: "memory");
+    return r;
+}
+struct RoundDownGuard {
+    int old;
+    RoundDownGuard() : old(std::fegetround()) { std::fesetround(FE_DOWNWARD); }
+    ~RoundDownGuard() { std::fesetround(old); }
+};
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+constexpr std::int64_t kMaxB = INT64_C(9223372036854775807);
+constexpr std::int64_t kMinB = -kMaxB;
+
+struct IV { double lb, ub; bool und; };
+struct IB { bool def, poss; };
+
+// compile-time bound classification helpers (fixed_point_bounds.hpp:72-180)
+struct CT { std::int64_t lb, ub; };
+inline bool ct_finite(CT b) { return b.lb > kMinB && b.ub < kMaxB; }
+inline bool nonneg(CT b, const IV& f) { if(b.lb >= 0) return true; if(b.ub <= 0) return false; return f.lb >= 0; }
+inline bool nonpos(CT b, const IV& f) { if(b.ub <= 0) return true; if(b.lb >= 0) return false; return f.ub <= 0; }
+inline bool positive(CT b, const IV& f) { if(b.lb > 0) return true; if(b.ub <= 0) return false; return f.lb > 0; }
+inline bool negative(CT b, const IV& f) { if(b.ub < 0) return true; if(b.lb >= 0) return false; return f.ub < 0; }
+const CT kUnbounded{kMinB, kMaxB};
+
+inline double nan_to_zero(double v) { return v != v ? 0.0 : v; }
+inline void do_mul(double l1, double l2, double u1, double u2, bool finite, double& lo, double& hi) {
+    lo = rd_mul(l1, l2);
+    hi = ru_mul(u1, u2);
+    if(!finite) { lo = nan_to_zero(lo); hi = nan_to_zero(hi); }
+}
+IV mul_check_signs(const IV& a, const IV& b, CT B1, CT B2) {
+    const bool fin = ct_finite(B1) && ct_finite(B2);
+    double lo, hi;
+    if(nonneg(B1, a)) {
+        if(nonneg(B2, b)) do_mul(a.lb, b.lb, a.ub, b.ub, fin, lo, hi);
+        else if(nonpos(B2, b)) do_mul(a.ub, b.lb, a.lb, b.ub, fin, lo, hi);
+        else do_mul(a.ub, b.lb, a.ub, b.ub, fin, lo, hi);
+    } else if(nonpos(B1, a)) {
+        if(nonneg(B2, b)) do_mul(a.lb, b.ub, a.ub, b.lb, fin, lo, hi);
+        else if(nonpos(B2, b)) do_mul(a.ub, b.ub, a.lb, b.lb, fin, lo, hi);
+        else do_mul(a.lb, b.ub, a.lb, b.lb, fin, lo, hi);
+    } else if(nonneg(B2, b)) {
+        do_mul(a.lb, b.ub, a.ub, b.ub, fin, lo, hi);
+    } else if(nonpos(B2, b)) {
+        do_mul(a.ub, b.lb, a.lb, b.lb, fin, lo, hi);
+    } else {
+        double l1, h1, l2, h2;
+        do_mul(a.lb, b.ub, a.lb, b.lb, fin, l1, h1);
+        do_mul(b.lb, a.ub, a.ub, b.ub, fin, l2, h2);
+        lo = (l2 <= l1) ? l2 : l1;
+        hi = (h1 <= h2) ? h2 : h1;
+    }
+    return IV{lo, hi, a.und || b.und};
+}
+
+inline int sgn(double v) { return (v > 0) - (v < 0); }
+void divide_by_zero(IV& num, const IV& den, int sl, int su, int dsum) {
+    num.und = true;
+    if(dsum == 0 || su - sl == 2) { num.lb = -kInf; num.ub = kInf; return; }
+    if(dsum > 0) {
+        if(sl >= 0) { num.lb = rd_div(num.lb, den.ub); num.ub = kInf; }
+        else { num.ub = ru_div(num.ub, den.ub); num.lb = -kInf; }
+    } else {
+        if(sl >= 0) { num.ub = ru_div(num.lb, den.lb); num.lb = -kInf; }
+        else { num.lb = rd_div(num.ub, den.lb); num.ub = kInf; }
+    }
+}
+struct WInf { double v; int inf_sign; };
+inline double div_lb_inf(WInf n, WInf d) {
+    if(!n.inf_sign && !d.inf_sign) return rd_div(n.v, d.v);
+    if(d.inf_sign != 0) return 0.0;
+    return -kInf;
+}
+inline double div_ub_inf(WInf n, WInf d) {
+    if(!n.inf_sign && !d.inf_sign) return ru_div(n.v, d.v);
+    if(d.inf_sign != 0) return 0.0;
+    return kInf;
+}
+void divide_by_infinite(IV& num, const IV& den) {
+    num.und = num.und || den.und;
+    int snl = sgn(num.lb), snu = sgn(num.ub), sdl = sgn(den.lb), sdu = sgn(den.ub);
+    WInf nl{num.lb, std::isfinite(num.lb) ? 0 : -1}, nu{num.ub, std::isfinite(num.ub) ? 0 : 1};
+    WInf dl{den.lb, std::isfinite(den.lb) ? 0 : -1}, du{den.ub, std::isfinite(den.ub) ? 0 : 1};
+    auto set = [&](WInf a, WInf b, WInf c, WInf d) { double l = div_lb_inf(a, b), u = div_ub_inf(c, d); num.lb = l; num.ub = u; };
+    if(sdl > 0) {
+        if(snl >= 0) set(nl, du, nu, dl);
+        else if(snu <= 0) set(nl, dl, nu, du);
+        else set(nl, dl, nu, dl);
+    } else if(sdu < 0) {
+        if(snl >= 0) set(nu, du, nl, dl);
+        else if(snu <= 0) set(nu, dl, nl, du);
+        else set(nu, du, nl, du);
+    } else {
+        divide_by_zero(num, den, snl, snu, sdl + sdu);
+    }
+}
+IV interval_div(IV num, const IV& den, CT BN, CT BD) {
+    bool fin_ct = ct_finite(BN) && ct_finite(BD);
+    bool fin_rt = std::isfinite(num.lb) && std::isfinite(num.ub) && std::isfinite(den.lb) && std::isfinite(den.ub);
+    if(!(fin_ct || fin_rt)) { divide_by_infinite(num, den); return num; }
+    num.und = num.und || den.und;
+    auto ia = [&](double a, double b, double c, double d) { double l = rd_div(a, b), u = ru_div(c, d); num.lb = l; num.ub = u; };
+    if(positive(BD, den)) {
+        if(nonneg(BN, num)) ia(num.lb, den.ub, num.ub, den.lb);
+        else if(nonpos(BN, num)) ia(num.lb, den.lb, num.ub, den.ub);
+        else ia(num.lb, den.lb, num.ub, den.lb);
+    } else if(negative(BD, den)) {
+        if(nonneg(BN, num)) ia(num.ub, den.ub, num.lb, den.lb);
+        else if(nonpos(BN, num)) ia(num.ub, den.lb, num.lb, den.ub);
+        else ia(num.ub, den.ub, num.lb, den.ub);
+    } else {
+        divide_by_zero(num, den, sgn(num.lb), sgn(num.ub), sgn(den.lb) + sgn(den.ub));
+    }
+    return num;
+}
+IV interval_sqrt(const IV& x, CT B) {
+    double lb = x.lb, ub = x.ub;
+    bool und = x.und;
+    if(!nonneg(B, x)) {
+        if(ub < 0) return IV{-kInf, kInf, true};
+        lb = 0;
+        und = true;
+    }
+    return IV{rd_sqrt(lb), ru_sqrt(ub), und};
+}
+inline double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }
+inline double dmin(double n1, double n2) { return n1 < n2 ? n1 : n2; }
+
+inline IB less(const IV& a, const IV& b) {
+    return IB{!a.und && !b.und && a.ub < b.lb, a.und || b.und || a.lb < b.ub};
+}
+inline IB inot(IB x) { return IB{!x.poss, !x.def}; }
+
+// ---------------------------------------------------------------- program
+enum Op { C_CONST, C_ARG, C_NEG, C_ADD, C_SUB, C_MUL, C_DIV, C_SQRT, C_SQUARE, C_MAX, C_MIN, C_ITE, C_ACOS,
+          C_BCONST, C_LT, C_LE, C_GT, C_GE, C_EQ, C_NEQ, C_NOT, C_OR, C_AND };
+const char* const kOpNames[] = {"const", "arg", "neg", "add", "sub", "mul", "div", "sqrt", "square", "max", "min",
+                                "ite", "acos", "bconst", "lt", "le", "gt", "ge", "eq", "neq", "not", "or", "and"};
+
+struct PNode {
+    int op;
+    std::vector<int> kids;
+    int arg;
+    double c_lb, c_ub;
+    bool c_undef, b_def, b_poss;
+    CT ct;
+};
+struct PBound { int target, dir, expr; bool honor; };
+struct PSplit { int arg; std::uint32_t initial, subs; bool dynamic; };
+
+struct Program {
+    int N = 0;
+    std::vector<PSplit> splits;
+    std::vector<PNode> nodes;
+    std::vector<PBound> bounds;
+    std::vector<std::vector<std::vector<int>>> update;
+    std::vector<std::vector<int>> rows;
+    std::vector<double> box_lb, box_ub;
+    std::vector<std::vector<int>> bound_order;  // nodes needed per bound, topological
+    std::vector<std::vector<int>> pred_order;   // nodes needed per node (for constraint roots)
+};
+
+double from_bits(std::uint64_t u) { double d; std::memcpy(&d, &u, 8); return d; }
+
+void collect(const Program& p, int root, std::vector<int>& out, std::vector<char>& seen) {
+    if(seen[static_cast<std::size_t>(root)]) return;
+    seen[static_cast<std::size_t>(root)] = 1;
+    for(int k : p.nodes[static_cast<std::size_t>(root)].kids) collect(p, k, out, seen);
+    out.push_back(root);
+}
+
+Program parse(const std::string& text) {
+    Program p;
+    std::istringstream in(text);
+    std::string line;
+    if(!std::getline(in, line) || line != "sqd-program 1") throw std::runtime_error("bad program header");
+    bool ended = false;
+    while(std::getline(in, line)) {
+        std::istringstream s(line);
+        std::string key;
+        s >> key;
+        if(key == "end") { ended = true; break; }
+        if(key == "nargs") {
+            s >> p.N;
+            p.update.assign(static_cast<std::size_t>(p.N), std::vector<std::vector<int>>(3));
+            p.rows.assign(static_cast<std::size_t>(p.N), {});
+            p.box_lb.assign(static_cast<std::size_t>(p.N), 0);
+            p.box_ub.assign(static_cast<std::size_t>(p.N), 0);
+        } else if(key == "split") {
+            PSplit sp; int d; s >> sp.arg >> sp.initial >> sp.subs >> d; sp.dynamic = d != 0; p.splits.push_back(sp);
+        } else if(key == "node") {
+            PNode n; std::size_t id, nk; std::string op; std::uint64_t cl, cu; int cund, bd, bp, pl, pu;
+            s >> id >> op >> n.arg >> nk;
+            n.op = -1;
+            for(int i = 0; i < 23; ++i) if(op == kOpNames[i]) n.op = i;
+            if(n.op < 0) throw std::runtime_error("unknown op " + op);
+            n.kids.resize(nk);
+            for(auto& k : n.kids) s >> k;
+            s >> cl >> cu >> cund >> bd >> bp >> n.ct.lb >> n.ct.ub >> pl >> pu;
+            n.c_lb = from_bits(cl); n.c_ub = from_bits(cu);
+            n.c_undef = cund != 0; n.b_def = bd != 0; n.b_poss = bp != 0;
+            if(id != p.nodes.size()) throw std::runtime_error("node order");
+            p.nodes.push_back(n);
+        } else if(key == "bound") {
+            PBound b; std::size_t k; int hon; s >> k >> b.target >> b.dir >> b.expr >> hon; b.honor = hon != 0;
+            p.bounds.push_back(b);
+        } else if(key == "update") {
+            int a, w; std::size_t n; s >> a >> w >> n;
+            auto& l = p.update.at(static_cast<std::size_t>(a)).at(static_cast<std::size_t>(w));
+            l.resize(n); for(auto& x : l) s >> x;
+        } else if(key == "row") {
+            int a; std::size_t n; s >> a >> n;
+            auto& r = p.rows.at(static_cast<std::size_t>(a));
+            r.resize(n); for(auto& x : r) s >> x;
+        } else if(key == "box") {
+            int a; std::uint64_t l, u; s >> a >> l >> u;
+            p.box_lb.at(static_cast<std::size_t>(a)) = from_bits(l);
+            p.box_ub.at(static_cast<std::size_t>(a)) = from_bits(u);
+        }
+        if(s.fail()) throw std::runtime_error("malformed line: " + line);
+    }
+    if(!ended) throw std::runtime_error("program text is truncated");
+    for(const auto& b : p.bounds) {
+        std::vector<int> o; std::vector<char> seen(p.nodes.size(), 0);
+        collect(p, b.expr, o, seen);
+        p.bound_order.push_back(o);
+    }
+    p.pred_order.resize(p.nodes.size());
+    for(const auto& r : p.rows) for(int c : r) {
+        if(!p.pred_order[static_cast<std::size_t>(c)].empty()) continue;
+        std::vector<int> o; std::vector<char> seen(p.nodes.size(), 0);
+        collect(p, c, o, seen);
+        p.pred_order[static_cast<std::size_t>(c)] = o;
+    }
+    for(const auto& n : p.nodes) if(n.op == C_ACOS) throw std::runtime_error("acos is not supported by the oracle yet");
+    return p;
+}
+
+// ---------------------------------------------------------------- evaluation
+struct Cuboid {
+    std::vector<double> lb, ub;
+    std::uint64_t depth = 0;
+};
+
+struct Evaluator {
+    const Program& p;
+    std::vector<IV> v;
+    std::vector<IB> pb;
+    explicit Evaluator(const Program& prog) : p(prog), v(prog.nodes.size()), pb(prog.nodes.size()) {}
+
+    void eval_node(int id, const Cuboid& c) {
+        const PNode& n = p.nodes[static_cast<std::size_t>(id)];
+        auto K = [&](int i) -> const IV& { return v[static_cast<std::size_t>(n.kids[static_cast<std::size_t>(i)])]; };
+        auto KP = [&](int i) -> IB { return pb[static_cast<std::size_t>(n.kids[static_cast<std::size_t>(i)])]; };
+        auto KCT = [&](int i) -> CT { return p.nodes[static_cast<std::size_t>(n.kids[static_cast<std::size_t>(i)])].ct; };
+        IV& r = v[static_cast<std::size_t>(id)];
+        switch(n.op) {
+            case C_CONST: r = IV{n.c_lb, n.c_ub, n.c_undef}; return;
+            case C_ARG: r = IV{c.lb[static_cast<std::size_t>(n.arg)], c.ub[static_cast<std::size_t>(n.arg)], false}; return;
+            case C_NEG: r = IV{-K(0).ub, -K(0).lb, K(0).und}; return;
+            case C_ADD: r = IV{rd_add(K(0).lb, K(1).lb), ru_add(K(0).ub, K(1).ub), K(0).und || K(1).und}; return;
+            case C_SUB: r = IV{rd_sub(K(0).lb, K(1).ub), ru_sub(K(0).ub, K(1).lb), K(0).und || K(1).und}; return;
+            case C_MUL: r = mul_check_signs(K(0), K(1), KCT(0), KCT(1)); return;
+            case C_DIV: r = interval_div(K(0), K(1), KCT(0), KCT(1)); return;
+            case C_SQRT: r = interval_sqrt(K(0), KCT(0)); return;
+            case C_SQUARE: {
+                IV x = mul_check_signs(K(0), K(0), kUnbounded, kUnbounded);
+                if(x.lb < 0) x.lb = 0;
+                r = x;
+                return;
+            }
+            case C_MAX: case C_MIN: {
+                std::size_t m = n.kids.size();
+                IV acc = v[static_cast<std::size_t>(n.kids[m - 1])];
+                for(std::size_t i = m - 1; i-- > 0;) {
+                    const IV& k = v[static_cast<std::size_t>(n.kids[i])];
+                    if(n.op == C_MAX) acc = IV{dmax(k.lb, acc.lb), dmax(k.ub, acc.ub), k.und || acc.und};
+                    else acc = IV{dmin(k.lb, acc.lb), dmin(k.ub, acc.ub), k.und || acc.und};
+                }
+                r = acc;
+                return;
+            }
+            case C_ITE: {
+                IB cnd = KP(0);
+                if(cnd.def) { r = K(1); return; }
+                if(!cnd.poss) { r = K(2); return; }
+                IV t = K(1);
+                const IV& e = K(2);
+                if(e.lb < t.lb) t.lb = e.lb;
+                if(e.ub > t.ub) t.ub = e.ub;
+                t.und = t.und || e.und;
+                r = t;
+                return;
+            }
+            case C_BCONST: pb[static_cast<std::size_t>(id)] = IB{n.b_def, n.b_poss}; return;
+            case C_LT: pb[static_cast<std::size_t>(id)] = less(K(0), K(1)); return;
+            case C_GT: pb[static_cast<std::size_t>(id)] = less(K(1), K(0)); return;
+            case C_GE: pb[static_cast<std::size_t>(id)] = inot(less(K(0), K(1))); return;
+            case C_LE: pb[static_cast<std::size_t>(id)] = inot(less(K(1), K(0))); return;
+            case C_EQ: case C_NEQ: {
+                const IV& a = K(0); const IV& b = K(1);
+                IB e{!a.und && !b.und && a.lb == a.ub && b.lb == b.ub && a.lb == b.lb,
+                     a.und || b.und || (!(a.ub < b.lb) && !(b.ub < a.lb))};
+                pb[static_cast<std::size_t>(id)] = n.op == C_EQ ? e : inot(e);
+                return;
+            }
+            case C_NOT: pb[static_cast<std::size_t>(id)] = inot(KP(0)); return;
+            case C_OR: case C_AND: {
+                IB acc = n.op == C_OR ? IB{false, false} : IB{true, true};
+                for(int k : n.kids) {
+                    IB x = pb[static_cast<std::size_t>(k)];
+                    if(n.op == C_OR) { acc.def = acc.def || x.def; acc.poss = acc.poss || x.poss; }
+                    else { acc.def = acc.def && x.def; acc.poss = acc.poss && x.poss; }
+                }
+                pb[static_cast<std::size_t>(id)] = acc;
+                return;
+            }
+            default: throw std::runtime_error("unsupported node");
+        }
+    }
+    const IV& eval_bound(int k, const Cuboid& c) {
+        for(int id : p.bound_order[static_cast<std::size_t>(k)]) eval_node(id, c);
+        return v[static_cast<std::size_t>(p.bounds[static_cast<std::size_t>(k)].expr)];
+    }
+    bool possibly(int pred, const Cuboid& c) {
+        for(int id : p.pred_order[static_cast<std::size_t>(pred)]) eval_node(id, c);
+        return pb[static_cast<std::size_t>(pred)].poss;
+    }
+};
+
+enum BoundID { B_NONE = 0, B_LB = 1, B_UB = 2, B_BOTH = 3, B_EMPTY = 4 };
+
+int apply_bound(Evaluator& ev, int k, Cuboid& c) {
+    const PBound& b = ev.p.bounds[static_cast<std::size_t>(k)];
+    IV iv = ev.eval_bound(k, c);
+    if(b.honor && iv.und) return B_NONE;
+    double& tl = c.lb[static_cast<std::size_t>(b.target)];
+    double& tu = c.ub[static_cast<std::size_t>(b.target)];
+    if(b.dir == 1) {
+        if(iv.ub < tu) { tu = iv.ub; return iv.ub < tl ? B_EMPTY : B_UB; }
+    } else if(b.dir == 2) {
+        if(iv.lb > tl) { tl = iv.lb; return iv.lb > tu ? B_EMPTY : B_LB; }
+    } else {
+        int lbc = B_NONE, ubc = B_NONE;
+        if(iv.ub < tu) { tu = iv.ub; ubc = B_UB; }
+        if(iv.lb > tl) { tl = iv.lb; lbc = B_LB; }
+        if(tu < tl) return B_EMPTY;
+        return lbc | ubc;
+    }
+    return B_NONE;
+}
+
+// Fixed-capacity ring queue with membership set; empty iff begin == end, so a
+// queue holding every bound index at once reads as empty (index_queue_set.hpp:62-91).
+struct IndexQueue {
+    std::size_t cap;
+    std::size_t begin = 0, end = 0;
+    std::vector<int> elems;
+    std::vector<char> present;
+    explicit IndexQueue(std::size_t n) : cap(n), elems(n), present(n, 0) {}
+    bool empty() const { return cap == 0 || begin == end; }
+    bool enqueue(int e) {
+        if(cap == 0) return false;
+        if(present[static_cast<std::size_t>(e)]) return false;
+        present[static_cast<std::size_t>(e)] = 1;
+        elems[end] = e;
+        if(++end >= cap) end = 0;
+        return true;
+    }
+    int dequeue() {
+        int e = elems[begin];
+        present[static_cast<std::size_t>(e)] = 0;
+        if(++begin >= cap) begin = 0;
+        return e;
+    }
+};
+
+bool propagate(Evaluator& ev, Cuboid& c, int arg, std::size_t it_limit, unsigned rec_limit) {
+    const Program& p = ev.p;
+    const std::size_t nb = p.bounds.size();
+    IndexQueue q(nb);
+    std::vector<unsigned> rec(nb, 0);
+    std::vector<char> unstable(nb, 0);
+    std::vector<int> unstable_list;
+    for(int k : p.update[static_cast<std::size_t>(arg)][2]) q.enqueue(k);
+    auto handle = [&](const std::vector<int>& lst) {
+        for(int a : lst) {
+            if(rec[static_cast<std::size_t>(a)] >= rec_limit) {
+                if(!q.present[static_cast<std::size_t>(a)] && !unstable[static_cast<std::size_t>(a)]) {
+                    unstable_list.push_back(a);
+                    unstable[static_cast<std::size_t>(a)] = 1;
+                }
+            } else if(q.enqueue(a)) {
+                rec[static_cast<std::size_t>(a)] += 1;
+            }
+        }
+    };
+    for(std::size_t i = 0; i < it_limit; ++i) {
+        while(!q.empty()) {
+            int k = q.dequeue();
+            int r = apply_bound(ev, k, c);
+            int t = p.bounds[static_cast<std::size_t>(k)].target;
+            switch(r) {
+                case B_EMPTY: return true;
+                case B_LB: handle(p.update[static_cast<std::size_t>(t)][0]); break;
+                case B_UB: handle(p.update[static_cast<std::size_t>(t)][1]); break;
+                case B_BOTH: handle(p.update[static_cast<std::size_t>(t)][2]); break;
+                default: break;
+            }
+        }
+        if(!unstable_list.empty()) {
+            for(int a : unstable_list) q.enqueue(a);
+            std::fill(rec.begin(), rec.end(), 0u);
+            std::fill(unstable.begin(), unstable.end(), 0);
+            unstable_list.clear();
+        } else {
+            break;
+        }
+    }
+    return false;
+}
+
+struct Splitter {
+    double lb, ub, w;
+    int n;
+    Splitter(double l, double u, int cnt) : lb(l), ub(u), n(cnt) { w = rd_div(rd_sub(u, l), static_cast<double>(cnt)); }
+    double point(int i) const { return i >= n ? ub : rd_add(lb, rd_mul(static_cast<double>(i), w)); }
+};
+
+struct Counts {
+    std::uint64_t cuboids = 0, leaves = 0, criticals = 0, repeated = 0;
+    Counts& operator+=(const Counts& o) { cuboids += o.cuboids; leaves += o.leaves; criticals += o.criticals; repeated += o.repeated; return *this; }
+};
+
+struct Core {
+    const Program& p;
+    std::vector<int> static_splits;   // indices into p.splits
+    std::vector<int> dynamic_splits;
+    explicit Core(const Program& prog) : p(prog) {
+        for(std::size_t i = 0; i < prog.splits.size(); ++i) {
+            if(prog.splits[i].dynamic) dynamic_splits.push_back(static_cast<int>(i));
+            else static_splits.push_back(static_cast<int>(i));
+        }
+    }
+    bool satisfies_all(Evaluator& ev, const Cuboid& c) const {
+        for(int cn : p.rows[static_cast<std::size_t>(p.N - 1)]) if(!ev.possibly(cn, c)) return false;
+        return true;
+    }
+    bool violates_rows(Evaluator& ev, const Cuboid& c, int from, int to) const {
+        for(int r = from; r < to; ++r)
+            for(int cn : p.rows[static_cast<std::size_t>(r)]) if(!ev.possibly(cn, c)) return true;
+        return false;
+    }
+    // Static enumeration (cpu_prover_core.hpp:221-256); `leaf` is called for each surviving leaf.
+    template<typename Leaf>
+    void enumerate(Evaluator& ev, const Cuboid& e, std::size_t level, Counts& cnt, Leaf&& leaf) const {
+        if(level == static_splits.size()) { cnt.leaves++; leaf(e); return; }
+        const PSplit& s = p.splits[static_cast<std::size_t>(static_splits[level])];
+        const int arg = s.arg;
+        const int subs = static_cast<int>(s.subs);
+        const int row_end = level + 1 < static_splits.size()
+                            ? p.splits[static_cast<std::size_t>(static_splits[level + 1])].arg : p.N;
+        Splitter sp(e.lb[static_cast<std::size_t>(arg)], e.ub[static_cast<std::size_t>(arg)], subs);
+        for(int i = 0; i < subs; ++i) {
+            Cuboid q = e;
+            q.lb[static_cast<std::size_t>(arg)] = sp.point(i);
+            q.ub[static_cast<std::size_t>(arg)] = sp.point(i + 1);
+            if(propagate(ev, q, arg, static_cast<std::size_t>(p.N), 4)) { cnt.leaves++; continue; }
+            if(violates_rows(ev, q, arg, row_end)) { cnt.leaves++; continue; }
+            enumerate(ev, q, level + 1, cnt, leaf);
+        }
+        cnt.cuboids += static_cast<std::uint64_t>(subs);
+    }
+    // Non-final pass with max_iterations_per_node = 1: the hull of surviving leaves.
+    bool nonfinal(Evaluator& ev, const Cuboid& e, Cuboid& hull, Counts& cnt) const {
+        bool empty = true;
+        enumerate(ev, e, 0, cnt, [&](const Cuboid& leaf) {
+            if(empty) { hull = leaf; empty = false; return; }
+            for(int a = 0; a < p.N; ++a) {
+                std::size_t i = static_cast<std::size_t>(a);
+                if(leaf.lb[i] < hull.lb[i]) hull.lb[i] = leaf.lb[i];
+                if(leaf.ub[i] > hull.ub[i]) hull.ub[i] = leaf.ub[i];
+            }
+        });
+        if(!empty) hull.depth = e.depth;
+        return !empty;
+    }
+    template<typename Crit>
+    void final_pass(Evaluator& ev, const Cuboid& e, Counts& cnt, Crit&& crit) const {
+        enumerate(ev, e, 0, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); });
+    }
+    // Dynamic split (pd_impl_split_into.hpp:29-75) or initial split (proof_driver.hpp:320-391).
+    template<typename Out>
+    void split(Evaluator& ev, const Cuboid& e, std::size_t level, bool initial, Counts& cnt, Out&& out) const {
+        const std::size_t itlim = 2 * static_cast<std::size_t>(p.N);
+        const unsigned reclim = initial ? static_cast<unsigned>(std::min(p.N, 255)) : static_cast<unsigned>(std::min(p.N, 32));
+        if(level == dynamic_splits.size()) {
+            if(satisfies_all(ev, e)) out(e); else cnt.leaves++;
+            return;
+        }
+        const PSplit& s = p.splits[static_cast<std::size_t>(dynamic_splits[level])];
+        const int n = static_cast<int>(initial ? s.initial : s.subs);
+        Splitter sp(e.lb[static_cast<std::size_t>(s.arg)], e.ub[static_cast<std::size_t>(s.arg)], n);
+        for(int i = 0; i < n; ++i) {
+            cnt.cuboids++;
+            Cuboid q = e;
+            q.lb[static_cast<std::size_t>(s.arg)] = sp.point(i);
+            q.ub[static_cast<std::size_t>(s.arg)] = sp.point(i + 1);
+            if(propagate(ev, q, s.arg, itlim, reclim)) { cnt.leaves++; continue; }
+            split(ev, q, level + 1, initial, cnt, out);
+        }
+    }
+};
+
+Cuboid initial_box(const Program& p) {
+    Cuboid c;
+    c.lb = p.box_lb;
+    c.ub = p.box_ub;
+    Evaluator ev(p);
+    for(int it = 0; it < 50 * p.N; ++it) {
+        for(std::size_t k = 0; k < p.bounds.size(); ++k)
+            if(apply_bound(ev, static_cast<int>(k), c) == B_EMPTY) break;
+    }
+    return c;
+}
+
+std::vector<Cuboid> initial_queue(const Program& p, Counts& cnt) {
+    Core core(p);
+    Evaluator ev(p);
+    std::vector<Cuboid> q;
+    Cuboid root = initial_box(p);
+    core.split(ev, root, 0, true, cnt, [&](const Cuboid& c) { q.push_back(c); });
+    return q;
+}
+
+// LIFO frontier shared by worker threads; counts do not depend on the schedule.
+struct Frontier {
+    std::mutex m;
+    std::condition_variable cv;
+    std::vector<Cuboid> stack;
+    int busy = 0;
+    bool pop(std::vector<Cuboid>& out, std::size_t batch) {
+        std::unique_lock<std::mutex> l(m);
+        cv.wait(l, [&] { return !stack.empty() || busy == 0; });
+        if(stack.empty()) return false;
+        for(std::size_t i = 0; i < batch && !stack.empty(); ++i) { out.push_back(std::move(stack.back())); stack.pop_back(); }
+        ++busy;
+        return true;
+    }
+    void push_done(std::vector<Cuboid>& items) {
+        std::unique_lock<std::mutex> l(m);
+        for(auto& c : items) stack.push_back(std::move(c));
+        items.clear();
+        --busy;
+        cv.notify_all();
+    }
+};
+
+struct RunResult {
+    Counts counts;
+    std::vector<Cuboid> criticals;
+};
+
+void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations, int threads, RunResult& res) {
+    Core core(p);
+    Frontier f;
+    f.stack = std::move(start);
+    std::mutex res_m;
+    std::vector<std::thread> ts;
+    std::vector<Counts> counts(static_cast<std::size_t>(threads));
+    std::vector<std::uint64_t> flops(static_cast<std::size_t>(threads), 0);
+    for(int t = 0; t < threads; ++t) {
+        ts.emplace_back([&, t] {
+            RoundDownGuard g;
+            t_flops = 0;
+            Evaluator ev(p);
+            Counts& cnt = counts[static_cast<std::size_t>(t)];
+            std::vector<Cuboid> batch, produced;
+            while(f.pop(batch, 32)) {
+                for(const Cuboid& e : batch) {
+                    Cuboid hull;
+                    if(!core.nonfinal(ev, e, hull, cnt)) continue;
+                    if(hull.depth >= static_cast<std::uint64_t>(generations)) {
+                        std::vector<Cuboid> crit;
+                        core.final_pass(ev, hull, cnt, [&](const Cuboid& c) { crit.push_back(c); });
+                        if(!crit.empty()) {
+                            std::lock_guard<std::mutex> l(res_m);
+                            for(auto& c : crit) res.criticals.push_back(c);
+                        }
+                    } else {
+                        hull.depth += 1;
+                        core.split(ev, hull, 0, false, cnt, [&](const Cuboid& c) { produced.push_back(c); });
+                    }
+                }
+                batch.clear();
+                f.push_done(produced);
+            }
+            flops[static_cast<std::size_t>(t)] = t_flops;
+        });
+    }
+    for(auto& t : ts) t.join();
+    for(const auto& c : counts) res.counts += c;
+    for(auto x : flops) g_flops += x;
+}
+
+thread_local std::string t_error;
+
+struct Handle {
+    Program prog;
+};
+
+template<typename F> int guarded(F&& f) {
+    try { f(); return 0; }
+    catch(const std::exception& e) { t_error = e.what(); return -1; }
+}
+
+void to_out(const Counts& c, OracleCounts* o) {
+    if(!o) return;
+    o->num_cuboids = c.cuboids;
+    o->num_leaf_cuboids = c.leaves;
+    o->num_critical_cuboids = c.criticals;
+    o->num_repeated_nodes = c.repeated;
+}
+
+Cuboid soa_get(const Program& p, std::uint64_t n, const double* lb, const double* ub, std::uint64_t i) {
+    Cuboid c;
+    c.lb.resize(static_cast<std::size_t>(p.N));
+    c.ub.resize(static_cast<std::size_t>(p.N));
+    for(int a = 0; a < p.N; ++a) {
+        c.lb[static_cast<std::size_t>(a)] = lb[static_cast<std::uint64_t>(a) * n + i];
+        c.ub[static_cast<std::size_t>(a)] = ub[static_cast<std::uint64_t>(a) * n + i];
+    }
+    return c;
+}
+
+} // namespace
+
+extern "C" {
+
+int oracle_load(const char* text, void** handle) {
+    return guarded([&] { auto* h = new Handle{parse(text)}; *handle = h; });
+}
+void oracle_free(void* handle) { delete static_cast<Handle*>(handle); }
+int oracle_nargs(void* handle) { return static_cast<Handle*>(handle)->prog.N; }
+const char* oracle_last_error(void) { return t_error.c_str(); }
+
+int oracle_initial_box(void* handle, double* lb, double* ub) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Cuboid c = initial_box(p);
+        for(int a = 0; a < p.N; ++a) { lb[a] = c.lb[static_cast<std::size_t>(a)]; ub[a] = c.ub[static_cast<std::size_t>(a)]; }
+    });
+}
+
+int oracle_initial_queue(void* handle, double* lb, double* ub, std::uint64_t cap, std::uint64_t* n, OracleCounts* counts) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Counts cnt;
+        std::vector<Cuboid> q = initial_queue(p, cnt);
+        *n = q.size();
+        for(std::uint64_t i = 0; i < q.size() && i < cap; ++i)
+            for(int a = 0; a < p.N; ++a) {
+                lb[i * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = q[i].lb[static_cast<std::size_t>(a)];
+                ub[i * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = q[i].ub[static_cast<std::size_t>(a)];
+            }
+        to_out(cnt, counts);
+    });
+}
+
+int oracle_run(void* handle, int generations, int threads, OracleCounts* counts,
+               double* criticals, std::uint64_t max_criticals, std::uint64_t* n_criticals) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Counts init;
+        std::vector<Cuboid> q = initial_queue(p, init);
+        // the reference pops the last element first; reverse-free order does not change counts
+        RunResult r;
+        run_from_queue(p, std::move(q), generations, threads < 1 ? 1 : threads, r);
+        r.counts += init;
+        to_out(r.counts, counts);
+        if(n_criticals) *n_criticals = r.criticals.size();
+        for(std::uint64_t i = 0; criticals && i < r.criticals.size() && i < max_criticals; ++i)
+            for(int a = 0; a < p.N; ++a) {
+                criticals[(i * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)) * 2] = r.criticals[i].lb[static_cast<std::size_t>(a)];
+                criticals[(i * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)) * 2 + 1] = r.criticals[i].ub[static_cast<std::size_t>(a)];
+            }
+    });
+}
+
+int oracle_run_slice(void* handle, int generations, int threads, std::uint64_t first, std::uint64_t last,
+                     OracleCounts* counts, std::uint64_t* initial_queue_len) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Counts init;
+        std::vector<Cuboid> q = initial_queue(p, init);
+        if(initial_queue_len) *initial_queue_len = q.size();
+        if(last > q.size()) last = q.size();
+        if(first > last) first = last;
+        std::vector<Cuboid> part(q.begin() + static_cast<std::ptrdiff_t>(first), q.begin() + static_cast<std::ptrdiff_t>(last));
+        RunResult r;
+        run_from_queue(p, std::move(part), generations, threads < 1 ? 1 : threads, r);
+        to_out(r.counts, counts);
+    });
+}
+
+int oracle_nonfinal(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                    double* hull_lb, double* hull_ub, std::uint8_t* has_hull, OracleCounts* counts) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Core core(p);
+        Evaluator ev(p);
+        Counts cnt;
+        for(std::uint64_t i = 0; i < n; ++i) {
+            Cuboid e = soa_get(p, n, lb, ub, i), h;
+            bool ok = core.nonfinal(ev, e, h, cnt);
+            has_hull[i] = ok ? 1 : 0;
+            for(int a = 0; a < p.N; ++a) {
+                hull_lb[static_cast<std::uint64_t>(a) * n + i] = ok ? h.lb[static_cast<std::size_t>(a)] : 0.0;
+                hull_ub[static_cast<std::uint64_t>(a) * n + i] = ok ? h.ub[static_cast<std::size_t>(a)] : 0.0;
+            }
+        }
+        to_out(cnt, counts);
+    });
+}
+
+int oracle_final(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                 double* crit_lb, double* crit_ub, std::uint64_t cap, std::uint64_t* n_crit, OracleCounts* counts) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Core core(p);
+        Evaluator ev(p);
+        Counts cnt;
+        std::uint64_t k = 0;
+        for(std::uint64_t i = 0; i < n; ++i) {
+            Cuboid e = soa_get(p, n, lb, ub, i);
+            core.final_pass(ev, e, cnt, [&](const Cuboid& c) {
+                if(k < cap)
+                    for(int a = 0; a < p.N; ++a) {
+                        crit_lb[k * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = c.lb[static_cast<std::size_t>(a)];
+                        crit_ub[k * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = c.ub[static_cast<std::size_t>(a)];
+                    }
+                ++k;
+            });
+        }
+        *n_crit = k;
+        to_out(cnt, counts);
+    });
+}
+
+int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                 double* out_lb, double* out_ub, std::uint64_t cap, std::uint64_t* n_out, OracleCounts* counts) {
+    return guarded([&] {
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        Core core(p);
+        Evaluator ev(p);
+        Counts cnt;
+        std::uint64_t k = 0;
+        for(std::uint64_t i = 0; i < n; ++i) {
+            Cuboid e = soa_get(p, n, lb, ub, i);
+            core.split(ev, e, 0, false, cnt, [&](const Cuboid& c) {
+                if(k < cap)
+                    for(int a = 0; a < p.N; ++a) {
+                        out_lb[k * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = c.lb[static_cast<std::size_t>(a)];
+                        out_ub[k * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = c.ub[static_cast<std::size_t>(a)];
+                    }
+                ++k;
+            });
+        }
+        *n_out = k;
+        to_out(cnt, counts);
+    });
+}
+
+int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* lo, double* hi) {
+    return guarded([&] {
+        RoundDownGuard g;
+        for(std::uint64_t i = 0; i < n; ++i) {
+            switch(op) {
+                case 0: lo[i] = rd_add(a[i], b[i]); hi[i] = ru_add(a[i], b[i]); break;
+                case 1: lo[i] = rd_sub(a[i], b[i]); hi[i] = ru_sub(a[i], b[i]); break;
+                case 2: lo[i] = rd_mul(a[i], b[i]); hi[i] = ru_mul(a[i], b[i]); break;
+                case 3: lo[i] = rd_div(a[i], b[i]); hi[i] = ru_div(a[i], b[i]); break;
+                case 4: lo[i] = rd_sqrt(a[i]); hi[i] = ru_sqrt(a[i]); break;
+                default: throw std::invalid_argument("unknown op");
+            }
+        }
+    });
+}
+
+std::uint64_t oracle_flop_count(void) { return g_flops.load(); }
+void oracle_flop_reset(void) { g_flops = 0; }
+void oracle_flop_enable(int on) { g_count_flops = on != 0; }
+
+} // extern "C"

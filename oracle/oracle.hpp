@@ -1,0 +1,77 @@
+// CPU oracle: test infrastructure only.
+//
+// A plain C++ restatement of ivarp's CPU prover core (the reference's parity
+// oracle), run on the lowered program text.  It is used by the tests, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker and
+// the CPU baseline; nothing in the product links, loads or calls it.
+//
+// Restated reference functions (file:line):
+//   interval + - * / sqrt, min/max, comparisons, IBool
+//        ivarp/include/ivarp/number/float_interval_addsub.hpp:53-102,
+//        float_interval_mul.hpp:33-166, float_interval_div.hpp:45-107, interval_div.hpp:57-288,
+//        math_fn/interval_sqrt.hpp:72-116, number/minmax.hpp:59-177,
+//        number/interval_comparisons.hpp:32-290, bool.hpp:70-149,
+//        math_fn/fixed_pow.hpp:173-200, math_fn/eval/if_then_else.hpp:64-82
+//   bound application      refactor_constraint_system/constraints_to_bounds/compile_time_bound.hpp:42-85
+//   propagation            bound_propagation/iterated_recursive.hpp:40-131,
+//                          bound_propagation/dynamic_bound_application.hpp:59-153, index_queue_set.hpp:38-101
+//   splitting              splitter.hpp:36-121
+//   static enumeration,    cpu_prover/cpu_prover_core.hpp:52-256
+//   hull and criticals
+//   driver                 prover/proof_driver.hpp:300-438,472-509, prover/pd_impl_split_into.hpp:29-75
+//   initial box            refactor_constraint_system/refactor_constraints_impl.hpp:61-101
+#pragma once
+#include <cstdint>
+#include <cstddef>
+
+extern "C" {
+
+struct OracleCounts {
+    std::uint64_t num_cuboids;
+    std::uint64_t num_leaf_cuboids;
+    std::uint64_t num_critical_cuboids;
+    std::uint64_t num_repeated_nodes;
+};
+
+// Parse a lowered program (text form).  Returns 0 on success.
+int oracle_load(const char* program_text, void** handle);
+void oracle_free(void* handle);
+int oracle_nargs(void* handle);
+const char* oracle_last_error(void);
+
+// Initial runtime box: outward compile-time box plus 50*N rounds of all bounds.
+int oracle_initial_box(void* handle, double* lb, double* ub);
+
+// Whole proof with `threads` worker threads and the given generation count.
+// Criticals (up to max_criticals) are written as [lb0,ub0,lb1,ub1,...] per cuboid.
+int oracle_run(void* handle, int generations, int threads, OracleCounts* counts,
+               double* criticals, std::uint64_t max_criticals, std::uint64_t* n_criticals);
+
+// Same, restricted to the initial-queue index range [first, last) of the
+// compacted initial queue (queue order of the reference's proof initialisation).
+int oracle_run_slice(void* handle, int generations, int threads, std::uint64_t first, std::uint64_t last,
+                     OracleCounts* counts, std::uint64_t* initial_queue_len);
+
+// The initial queue (compacted), n entries of N intervals; returns the length.
+int oracle_initial_queue(void* handle, double* lb, double* ub, std::uint64_t cap, std::uint64_t* n, OracleCounts* counts);
+
+// One non-final pass of the core over n boxes (SoA: lb[a*n+i], ub[a*n+i]).
+// For every input: has_hull[i] and the joined hull of the surviving static leaves.
+int oracle_nonfinal(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                    double* hull_lb, double* hull_ub, std::uint8_t* has_hull, OracleCounts* counts);
+// One final pass: criticals are appended in enumeration order (SoA per critical, stride N).
+int oracle_final(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                 double* crit_lb, double* crit_ub, std::uint64_t cap, std::uint64_t* n_crit, OracleCounts* counts);
+// Dynamic split of n boxes (depth d -> d+1): surviving children appended (stride N).
+int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* ub,
+                 double* out_lb, double* out_ub, std::uint64_t cap, std::uint64_t* n_out, OracleCounts* counts);
+
+// Elementwise directed-rounding primitives for arithmetic parity tests.
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt.  out_lo = RD(a op b), out_hi = RU(a op b).
+int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* out_lo, double* out_hi);
+
+// Binary64 arithmetic operations (add/sub/mul/div/sqrt) executed since the last reset.
+std::uint64_t oracle_flop_count(void);
+void oracle_flop_reset(void);
+void oracle_flop_enable(int on);
+}

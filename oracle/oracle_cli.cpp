@@ -1,0 +1,49 @@
+// Command-line driver of the CPU oracle (test infrastructure only).
+//   sqd_oracle <program-file> [--threads T] [--generations G] [--flops] [--slice FIRST LAST]
+// Prints one JSON line with the counts, the wall time and the thread count.
+#include "oracle.hpp"
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <thread>
+
+int main(int argc, char** argv) {
+    if(argc < 2) { std::fprintf(stderr, "usage: sqd_oracle <program> [--threads T] [--generations G] [--flops] [--slice A B]\n"); return 2; }
+    int threads = static_cast<int>(std::thread::hardware_concurrency());
+    int gens = 8;
+    bool flops = false;
+    bool slice = false;
+    unsigned long long first = 0, last = 0;
+    for(int i = 2; i < argc; ++i) {
+        if(!std::strcmp(argv[i], "--threads") && i + 1 < argc) threads = std::atoi(argv[++i]);
+        else if(!std::strcmp(argv[i], "--generations") && i + 1 < argc) gens = std::atoi(argv[++i]);
+        else if(!std::strcmp(argv[i], "--flops")) flops = true;
+        else if(!std::strcmp(argv[i], "--slice") && i + 2 < argc) { slice = true; first = std::strtoull(argv[++i], nullptr, 10); last = std::strtoull(argv[++i], nullptr, 10); }
+    }
+    std::ifstream f(argv[1]);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    void* h = nullptr;
+    if(oracle_load(ss.str().c_str(), &h) != 0) { std::fprintf(stderr, "load: %s\n", oracle_last_error()); return 1; }
+    oracle_flop_enable(flops ? 1 : 0);
+    oracle_flop_reset();
+    OracleCounts c{};
+    std::uint64_t ncrit = 0, iql = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = slice ? oracle_run_slice(h, gens, threads, first, last, &c, &iql)
+                   : oracle_run(h, gens, threads, &c, nullptr, 0, &ncrit);
+    auto t1 = std::chrono::steady_clock::now();
+    if(rc != 0) { std::fprintf(stderr, "run: %s\n", oracle_last_error()); return 1; }
+    double s = std::chrono::duration<double>(t1 - t0).count();
+    std::printf("{\"num_cuboids\": %llu, \"num_leaf_cuboids\": %llu, \"num_critical_cuboids\": %llu, "
+                "\"seconds\": %.6f, \"threads\": %d, \"flops\": %llu, \"initial_queue\": %llu}\n",
+                static_cast<unsigned long long>(c.num_cuboids), static_cast<unsigned long long>(c.num_leaf_cuboids),
+                static_cast<unsigned long long>(c.num_critical_cuboids), s, threads,
+                static_cast<unsigned long long>(oracle_flop_count()), static_cast<unsigned long long>(iql));
+    oracle_free(h);
+    return 0;
+}
