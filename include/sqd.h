@@ -1,0 +1,136 @@
+/*
+ * sqd.h - C ABI of the MI355X (gfx950) branch-and-bound prover core.
+ *
+ * The reference's drop-in boundary is the duck-typed "prover core" consumed by
+ * ProofDriver (ivarp/include/ivarp/prover/proof_driver.hpp:218-627), implemented
+ * by CPUProverCore (cpu_prover/cpu_prover_core.hpp:31-260) and CUDAProverCore
+ * (with_cuda/ivarp_cuda/gpu_prover_core.cuh:199-475) and selected by run_prover
+ * (run_prover.hpp:57-74).  Each entry point below names the member it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative sqd_status; the message
+ *    of the last failure on the calling thread is available from sqd_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Boxes are passed in structure-of-arrays layout: for n boxes of N intervals,
+ *    lb[a * n + i] and ub[a * n + i] are the bounds of argument a of box i.
+ *  - Host buffers are caller-owned; device buffers are owned by the library.
+ *  - One sqd_core per device; a core is thread-compatible, not thread-safe
+ *    (like the reference, calls for one core are never concurrent).
+ *  - Counts follow the reference's CuboidCounts semantics
+ *    (prover/proof_driver.hpp:48-70, prover/proof_information.hpp:32-44).
+ */
+#ifndef SQD_H
+#define SQD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sqd_status {
+    SQD_OK = 0,
+    SQD_ERR_ARGUMENT = -1,   /* invalid argument or malformed program text */
+    SQD_ERR_NO_KERNELS = -2, /* no compiled kernels for this program */
+    SQD_ERR_DEVICE = -3,     /* HIP runtime error (launch, memory, device) */
+    SQD_ERR_CAPACITY = -4,   /* a device buffer was too small for the workload */
+    SQD_ERR_INTERNAL = -5
+} sqd_status;
+
+typedef struct sqd_program sqd_program;
+typedef struct sqd_core sqd_core;
+
+/* ProofInformation / CuboidCounts (prover/proof_information.hpp:32-44) */
+typedef struct sqd_counts {
+    uint64_t num_cuboids;
+    uint64_t num_leaf_cuboids;
+    uint64_t num_critical_cuboids;
+    uint64_t num_repeated_nodes;
+} sqd_counts;
+
+/* The subset of ProverSettings (prover_input.hpp:124-159) the GPU core uses,
+ * plus its own sizing knobs.  Fill with sqd_settings_default(); zero sizing
+ * fields take the defaults, generation_count is used as given. */
+typedef struct sqd_settings {
+    uint32_t generation_count;     /* generations of dynamic splits (reference default 8) */
+    uint32_t dequeue_buffer_size;  /* dynamic cuboids per round; default chosen per program */
+    uint64_t frontier_capacity;    /* cuboids the device frontier can hold; default 1<<24 */
+    uint64_t level_bytes;          /* bytes per level buffer; default 1<<30 */
+    uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
+    uint32_t grid_blocks;          /* blocks per level launch; default 8 per CU */
+    uint32_t reserved;
+} sqd_settings;
+
+/* Defaults: generation_count = 8, every sizing field 0 (chosen by the core). */
+void sqd_settings_default(sqd_settings* settings);
+
+/* ---------------------------------------------------------------- programs */
+
+/* Parse a lowered program (the text produced by the lemma lowering) and bind it
+ * to the kernels compiled for it.  Replaces prover_input<...>(system) plus the
+ * template instantiation of the core (prover_input.hpp:118-122). */
+int sqd_program_from_text(const char* text, sqd_program** out);
+/* Lower one of the built-in lemma statements ("top1", "top2", "one", "two",
+ * "three", "four", "five", "six", "seven", "more1", "more2"). */
+int sqd_program_from_lemma(const char* key, sqd_program** out);
+void sqd_program_destroy(sqd_program* program);
+/* Canonical text of the program; *len receives the size including the NUL. */
+int sqd_program_text(const sqd_program* program, char* buffer, size_t capacity, size_t* len);
+int sqd_program_nargs(const sqd_program* program);
+/* 1 if kernels are compiled into this library for the program, else 0. */
+int sqd_program_has_kernels(const sqd_program* program);
+/* Number of built-in lemmas and the key of lemma i. */
+int sqd_lemma_count(void);
+const char* sqd_lemma_key(int i);
+const char* sqd_lemma_title(int i);
+
+/* ---------------------------------------------------------------- cores */
+
+/* initialize / initialize_per_thread / set_* (prover/basic_prover_core.hpp:30-67,
+ * gpu_prover_core.cuh:241-263,427-464): allocate device state on `device`. */
+int sqd_core_create(int device, const sqd_program* program, const sqd_settings* settings, sqd_core** out);
+void sqd_core_destroy(sqd_core* core);
+
+/* Initial runtime box (refactor_constraints_impl.hpp:61-101): N lb then N ub. */
+int sqd_initial_box(sqd_core* core, double* lb, double* ub);
+
+/* Initial queue (ProofDriver::proof_init_*, proof_driver.hpp:320-391): up to
+ * `capacity` boxes in SoA with stride `capacity`; *n receives the queue length. */
+int sqd_initial_queue(sqd_core* core, double* lb, double* ub, size_t capacity, size_t* n, sqd_counts* counts);
+
+/* handle_cuboids_nonfinal (cpu_prover_core.hpp:52-60): for each of n boxes the
+ * hull of its surviving static leaves; has_hull[i] = 0 if none survived. */
+int sqd_nonfinal(sqd_core* core, const double* lb, const double* ub, size_t n,
+                 double* hull_lb, double* hull_ub, uint8_t* has_hull, sqd_counts* counts);
+
+/* handle_cuboids_final (cpu_prover_core.hpp:62-71): critical leaves of n boxes,
+ * up to `capacity` (SoA, stride capacity); *n_critical receives the total. */
+int sqd_final(sqd_core* core, const double* lb, const double* ub, size_t n,
+              double* crit_lb, double* crit_ub, size_t capacity, size_t* n_critical, sqd_counts* counts);
+
+/* split_into (pd_impl_split_into.hpp:29-75): children of n boxes that survive
+ * propagation and every constraint (SoA, stride capacity). */
+int sqd_split(sqd_core* core, const double* lb, const double* ub, size_t n,
+              double* out_lb, double* out_ub, size_t capacity, size_t* n_out, sqd_counts* counts);
+
+/* ProofDriver::run (proof_driver.hpp:472-489) with the frontier resident in HBM:
+ * returns the counts; criticals (up to capacity, SoA stride capacity) are copied
+ * out when crit_lb/crit_ub are non-NULL.  *seconds (optional) receives the time
+ * spent on the device loop. */
+int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_ub,
+              size_t capacity, size_t* n_critical, double* seconds);
+
+/* ---------------------------------------------------------------- tests */
+
+/* Elementwise directed rounding on the device (op: 0 add, 1 sub, 2 mul, 3 div,
+ * 4 sqrt): lo = RD(a op b), hi = RU(a op b).  Arithmetic parity checks. */
+int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi);
+
+const char* sqd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SQD_H */

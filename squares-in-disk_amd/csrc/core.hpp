@@ -1,0 +1,449 @@
+// Generic GPU prover-core kernels, instantiated once per lowered program.
+//
+// One lane handles one child cuboid ("one cuboid per lane").  A static level
+// kernel takes a list of parent boxes (structure-of-arrays in HBM), splits the
+// level's variable, runs interval constraint propagation and the constraint
+// rows, and stream-compacts the survivors with a wave ballot + prefix count into
+// the next level's list.  The last static level joins the surviving leaves into
+// the per-dynamic-cuboid hull with 64-bit atomics on order-preserving keys (the
+// critical-cuboid reduction).  The dynamic split (split_into) and the initial
+// queue are level kernels of the same shape over the dynamic variables.
+//
+// Reference semantics restated here:
+//   static enumeration / counts   ivarp/include/ivarp/cpu_prover/cpu_prover_core.hpp:179-256
+//   hull (AddToRange)             cpu_prover_core.hpp:97-134, number/interval.hpp:77-84
+//   dynamic split                 prover/pd_impl_split_into.hpp:29-75
+//   initial queue                 prover/proof_driver.hpp:320-391
+//   propagation                   bound_propagation/iterated_recursive.hpp:40-131,
+//                                 dynamic_bound_application.hpp:104-123, index_queue_set.hpp:38-101
+//   splitter                      splitter.hpp:44-59
+//   initial box                   refactor_constraint_system/refactor_constraints_impl.hpp:61-101
+//
+// A program struct P (generated from the lowered program) provides:
+//   N, NB (bounds), NC (constants), NDYN, NSTAT,
+//   dyn_arg(j), dyn_initial(j), dyn_subs(j), st_arg(j), st_subs(j)  (constexpr)
+//   apply(k, lb, ub, C)      -> -1 empty, 0 unchanged, else 1 + update-list id
+//   deps(list, f)            calls f(a) for each bound index a in update list `list`
+//   rows_violated<L>(lb, ub, C)   constraint rows of static level L
+//   all_satisfied(lb, ub, C)      every remaining constraint possibly holds
+#pragma once
+#include "interval.hpp"
+
+namespace sqd {
+namespace dev {
+
+constexpr int kBlock = 256;
+
+enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
+
+struct DeviceCounters {
+    unsigned long long cuboids;
+    unsigned long long leaves;
+    unsigned long long criticals;
+    unsigned long long overflow;
+};
+
+// A list of boxes in structure-of-arrays layout: lb[a * stride + i].
+struct BoxList {
+    double* lb;
+    double* ub;
+    unsigned* depth;     // SPLIT/INIT/stack lists
+    unsigned* origin;    // static levels: index of the dynamic cuboid within the round
+    unsigned* leaf;      // static levels: enumeration index within that cuboid's grid
+    unsigned long long stride;
+    unsigned long long cap;
+    unsigned long long* count;  // device counter of valid entries
+};
+
+struct LevelArgs {
+    BoxList in;
+    BoxList out;
+    unsigned long long in_base;        // first input entry
+    unsigned long long in_fixed_count; // when nonzero: number of inputs (count pointer ignored)
+    unsigned long long* hull;          // NONFINAL last level: [origin][arg][4] keys
+    DeviceCounters* counters;
+    const double* consts;
+    unsigned generations;
+};
+
+// ---------------------------------------------------------------- hull keys
+__device__ __forceinline__ unsigned long long order_key(double x) {
+    unsigned long long u = bits(x == 0.0 ? 0.0 : x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_order_key(unsigned long long k) {
+    return from_bits((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+}
+
+// ---------------------------------------------------------------- splitter
+__device__ __forceinline__ double split_width(double lb, double ub, int n) {
+    return rd_div(rd_sub(ub, lb), static_cast<double>(n));
+}
+__device__ __forceinline__ double split_point(double lb, double ub, double w, int i, int n) {
+    return i >= n ? ub : rd_add(lb, rd_mul(static_cast<double>(i), w));
+}
+
+// ---------------------------------------------------------------- propagation
+// Per-lane queue state: the ring of pending bound indices and the list of
+// "unstable" bounds live in LDS (one byte column per lane); membership masks and
+// the 4-bit recursion counters live in registers.
+template<class P>
+struct PropScratch {
+    unsigned char ring[P::NB > 0 ? P::NB : 1][kBlock];
+    unsigned char ulist[P::NB > 0 ? P::NB : 1][kBlock];
+};
+
+__device__ __forceinline__ unsigned rec_get(const unsigned long long (&rec)[3], int a) {
+    const unsigned long long w = a < 16 ? rec[0] : (a < 32 ? rec[1] : rec[2]);
+    return static_cast<unsigned>((w >> ((a & 15) * 4)) & 15ull);
+}
+__device__ __forceinline__ void rec_inc(unsigned long long (&rec)[3], int a) {
+    const unsigned long long inc = 1ull << ((a & 15) * 4);
+    if(a < 16) rec[0] += inc; else if(a < 32) rec[1] += inc; else rec[2] += inc;
+}
+
+template<class P>
+__device__ bool propagate(double (&lb)[P::N], double (&ub)[P::N], int arg, int it_limit, unsigned rec_limit,
+                          const double* C, PropScratch<P>& s)
+{
+    constexpr int NB = P::NB;
+    static_assert(NB <= 48, "at most 48 bounds (4-bit recursion counters in three words)");
+    if(NB == 0) return false;
+    const int lane = threadIdx.x;
+    int qb = 0, qe = 0;
+    unsigned long long present = 0, unstable = 0;
+    unsigned long long rec[3] = {0, 0, 0};
+    int nun = 0;
+    // A full ring reads as empty (begin == end), as in the reference's IndexQueueSet.
+    auto enqueue = [&](int e) -> bool {
+        if((present >> e) & 1ull) return false;
+        present |= 1ull << e;
+        s.ring[qe][lane] = static_cast<unsigned char>(e);
+        if(++qe >= NB) qe = 0;
+        return true;
+    };
+    P::deps(arg * 3 + 2, [&](int a) { enqueue(a); });
+    for(int it = 0; it < it_limit; ++it) {
+        while(qb != qe) {
+            const int k = s.ring[qb][lane];
+            present &= ~(1ull << k);
+            if(++qb >= NB) qb = 0;
+            const int r = P::apply(k, lb, ub, C);
+            if(r < 0) return true;
+            if(r > 0) {
+                P::deps(r - 1, [&](int a) {
+                    if(rec_get(rec, a) >= rec_limit) {
+                        if(!((present >> a) & 1ull) && !((unstable >> a) & 1ull)) {
+                            s.ulist[nun][lane] = static_cast<unsigned char>(a);
+                            ++nun;
+                            unstable |= 1ull << a;
+                        }
+                    } else if(enqueue(a)) {
+                        rec_inc(rec, a);
+                    }
+                });
+            }
+        }
+        if(nun > 0) {
+            for(int j = 0; j < nun; ++j) enqueue(s.ulist[j][lane]);
+            rec[0] = rec[1] = rec[2] = 0;
+            unstable = 0;
+            nun = 0;
+        } else {
+            break;
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ unsigned long long wave_append(bool flag, unsigned long long* counter) {
+    const unsigned long long mask = __ballot(flag);
+    const unsigned long long active = __ballot(1);
+    const int lane = __lane_id();
+    const int first = __ffsll(static_cast<long long>(active)) - 1;
+    unsigned long long base = 0;
+    if(lane == first && mask) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, first);
+    return base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+__device__ __forceinline__ void wave_add(unsigned long long v, unsigned long long* counter) {
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const unsigned long long active = __ballot(1);
+    const int first = __ffsll(static_cast<long long>(active)) - 1;
+    if(__lane_id() == first && v) atomicAdd(counter, v);
+}
+
+template<class P>
+__device__ __forceinline__ void load_consts(double* dst, const double* src) {
+    for(int i = threadIdx.x; i < 2 * P::NC; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ unsigned long long item_count(const BoxList& in, unsigned long long fixed) {
+    return fixed ? fixed : *in.count;
+}
+
+template<class P>
+__device__ __forceinline__ void load_box(const BoxList& l, unsigned long long i, double (&lb)[P::N], double (&ub)[P::N]) {
+#pragma unroll
+    for(int a = 0; a < P::N; ++a) {
+        lb[a] = l.lb[static_cast<unsigned long long>(a) * l.stride + i];
+        ub[a] = l.ub[static_cast<unsigned long long>(a) * l.stride + i];
+    }
+}
+template<class P>
+__device__ __forceinline__ void store_box(const BoxList& l, unsigned long long i, const double (&lb)[P::N], const double (&ub)[P::N]) {
+#pragma unroll
+    for(int a = 0; a < P::N; ++a) {
+        l.lb[static_cast<unsigned long long>(a) * l.stride + i] = lb[a];
+        l.ub[static_cast<unsigned long long>(a) * l.stride + i] = ub[a];
+    }
+}
+
+// ---------------------------------------------------------------- static levels
+// Level L of the static grid (modes NONFINAL and FINAL).
+template<class P, int MODE, int L>
+__global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
+    constexpr int N = P::N;
+    constexpr int ARG = P::st_arg(L);
+    constexpr int SUBS = P::st_subs(L);
+    constexpr bool LAST = (L == P::NSTAT - 1);
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ PropScratch<P> s_prop;
+    load_consts<P>(s_c, args.consts);
+    enter_round_down();
+    const unsigned long long n_items = item_count(args.in, args.in_fixed_count);
+    const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
+    const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // every lane of a wave runs the same number of loop trips, so the wave-wide
+    // ballots below always see all lanes of the wave
+    const unsigned long long trips = (total + stride - 1) / stride;
+    for(unsigned long long trip = 0; trip < trips; ++trip) {
+        const unsigned long long w = first + trip * stride;
+        const bool valid = w < total;
+        bool survive = false;
+        double lb[N], ub[N];
+        unsigned origin = 0, leaf = 0;
+        if(valid) {
+            const unsigned long long item = w / SUBS;
+            const int i = static_cast<int>(w % SUBS);
+            const unsigned long long src = args.in_base + item;
+            load_box<P>(args.in, src, lb, ub);
+            if(args.in.origin) { origin = args.in.origin[src]; leaf = args.in.leaf[src]; }
+            else { origin = static_cast<unsigned>(item); leaf = 0; }
+            leaf = leaf * SUBS + static_cast<unsigned>(i);
+            if(i == 0) c_cub += SUBS;
+            const double pl = lb[ARG], pu = ub[ARG];
+            const double wd = split_width(pl, pu, SUBS);
+            lb[ARG] = split_point(pl, pu, wd, i, SUBS);
+            ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
+            if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+                ++c_leaf;
+            } else if(P::template rows_violated<L>(lb, ub, s_c)) {
+                ++c_leaf;
+            } else if(LAST) {
+                ++c_leaf;
+                if(MODE == FINAL) { ++c_crit; survive = true; }
+                else {
+                    unsigned long long* rec = args.hull + static_cast<unsigned long long>(origin) * N * 4;
+#pragma unroll
+                    for(int a = 0; a < N; ++a) {
+                        atomicMin(rec + a * 4 + 0, order_key(lb[a]));
+                        atomicMax(rec + a * 4 + 1, order_key(ub[a]));
+                        if(lb[a] == 0.0) atomicMin(rec + a * 4 + 2, (static_cast<unsigned long long>(leaf) << 1) | (bits(lb[a]) >> 63));
+                        if(ub[a] == 0.0) atomicMin(rec + a * 4 + 3, (static_cast<unsigned long long>(leaf) << 1) | (bits(ub[a]) >> 63));
+                    }
+                }
+            } else {
+                survive = true;
+            }
+        }
+        if(!LAST || MODE == FINAL) {
+            const unsigned long long pos = wave_append(survive, args.out.count);
+            if(survive) {
+                if(pos < args.out.cap) {
+                    store_box<P>(args.out, pos, lb, ub);
+                    if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
+                } else {
+                    atomicAdd(&args.counters->overflow, 1ull);
+                }
+            }
+        }
+    }
+    wave_add(c_cub, &args.counters->cuboids);
+    wave_add(c_leaf, &args.counters->leaves);
+    if(MODE == FINAL) wave_add(c_crit, &args.counters->criticals);
+}
+
+// ---------------------------------------------------------------- dynamic levels
+// Level J of the dynamic split (SPLIT: per-generation subdivisions, INIT: the
+// initial queue).  The last level keeps children that possibly satisfy every
+// constraint and appends them to the output (the cuboid stack).
+template<class P, int MODE, int J>
+__global__ __launch_bounds__(kBlock) void k_dynamic_level(LevelArgs args) {
+    constexpr int N = P::N;
+    constexpr int ARG = P::dyn_arg(J);
+    constexpr int SUBS = MODE == INIT ? P::dyn_initial(J) : P::dyn_subs(J);
+    constexpr bool LAST = (J == P::NDYN - 1);
+    constexpr unsigned REC = MODE == INIT ? (N < 255 ? N : 255) : (N < 32 ? N : 32);
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ PropScratch<P> s_prop;
+    load_consts<P>(s_c, args.consts);
+    enter_round_down();
+    const unsigned long long n_items = item_count(args.in, args.in_fixed_count);
+    const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned long long trips = (total + stride - 1) / stride;
+    unsigned long long c_cub = 0, c_leaf = 0;
+    for(unsigned long long trip = 0; trip < trips; ++trip) {
+        const unsigned long long w = first + trip * stride;
+        bool survive = false;
+        double lb[N], ub[N];
+        unsigned depth = 0;
+        if(w < total) {
+            const unsigned long long item = w / SUBS;
+            const int i = static_cast<int>(w % SUBS);
+            const unsigned long long src = args.in_base + item;
+            load_box<P>(args.in, src, lb, ub);
+            depth = args.in.depth ? args.in.depth[src] : 0u;
+            ++c_cub;
+            const double pl = lb[ARG], pu = ub[ARG];
+            const double wd = split_width(pl, pu, SUBS);
+            lb[ARG] = split_point(pl, pu, wd, i, SUBS);
+            ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
+            if(propagate<P>(lb, ub, ARG, 2 * N, REC, s_c, s_prop)) {
+                ++c_leaf;
+            } else if(LAST && !P::all_satisfied(lb, ub, s_c)) {
+                ++c_leaf;
+            } else {
+                survive = true;
+            }
+        }
+        const unsigned long long pos = wave_append(survive, args.out.count);
+        if(survive) {
+            if(pos < args.out.cap) {
+                store_box<P>(args.out, pos, lb, ub);
+                if(args.out.depth) args.out.depth[pos] = depth;
+            } else {
+                atomicAdd(&args.counters->overflow, 1ull);
+            }
+        }
+    }
+    wave_add(c_cub, &args.counters->cuboids);
+    wave_add(c_leaf, &args.counters->leaves);
+}
+
+// ---------------------------------------------------------------- round glue
+// Turn the hull records of one round into boxes.  Hulls at depth >= generations
+// go to `fin` (final pass), the others to `spl` with depth + 1 (dynamic split).
+// Without static variables the popped cuboid itself is the only leaf.
+template<class P>
+__global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, unsigned long long base, unsigned long long n,
+                                                          const unsigned long long* hull, BoxList fin, BoxList spl,
+                                                          DeviceCounters* counters, unsigned generations) {
+    constexpr int N = P::N;
+    const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool have = false;
+    double lb[N], ub[N];
+    unsigned depth = 0;
+    if(d < n) {
+        depth = items.depth[base + d];
+        if(P::NSTAT == 0) {
+            load_box<P>(items, base + d, lb, ub);
+            have = true;
+        } else {
+            const unsigned long long* rec = hull + d * N * 4;
+            have = rec[0] != ~0ull;
+            if(have) {
+#pragma unroll
+                for(int a = 0; a < N; ++a) {
+                    lb[a] = from_order_key(rec[a * 4 + 0]);
+                    ub[a] = from_order_key(rec[a * 4 + 1]);
+                    if(lb[a] == 0.0) lb[a] = (rec[a * 4 + 2] & 1ull) ? -0.0 : 0.0;
+                    if(ub[a] == 0.0) ub[a] = (rec[a * 4 + 3] & 1ull) ? -0.0 : 0.0;
+                }
+            }
+        }
+    }
+    if(P::NSTAT == 0) wave_add(have ? 1ull : 0ull, &counters->leaves);
+    const bool to_final = have && depth >= generations;
+    const bool to_split = have && !to_final;
+    const unsigned long long pf = wave_append(to_final, fin.count);
+    const unsigned long long ps = wave_append(to_split, spl.count);
+    if(to_final) {
+        if(pf < fin.cap) { store_box<P>(fin, pf, lb, ub); fin.depth[pf] = depth; }
+        else atomicAdd(&counters->overflow, 1ull);
+    }
+    if(to_split) {
+        if(ps < spl.cap) { store_box<P>(spl, ps, lb, ub); spl.depth[ps] = depth + 1; }
+        else atomicAdd(&counters->overflow, 1ull);
+    }
+}
+
+// Decode hull records in input order (sqd_nonfinal): entry d of `out` holds the
+// hull of input d, has[d] = 0 when no leaf survived.
+template<class P>
+__global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned long long n, const unsigned long long* hull,
+                                                        BoxList out, unsigned char* has) {
+    constexpr int N = P::N;
+    const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(d >= n) return;
+    double lb[N], ub[N];
+    bool ok;
+    if(P::NSTAT == 0) {
+        load_box<P>(items, d, lb, ub);
+        ok = true;
+    } else {
+        const unsigned long long* rec = hull + d * N * 4;
+        ok = rec[0] != ~0ull;
+#pragma unroll
+        for(int a = 0; a < N; ++a) {
+            lb[a] = ok ? from_order_key(rec[a * 4 + 0]) : 0.0;
+            ub[a] = ok ? from_order_key(rec[a * 4 + 1]) : 0.0;
+            if(ok && lb[a] == 0.0) lb[a] = (rec[a * 4 + 2] & 1ull) ? -0.0 : 0.0;
+            if(ok && ub[a] == 0.0) ub[a] = (rec[a * 4 + 3] & 1ull) ? -0.0 : 0.0;
+        }
+    }
+    store_box<P>(out, d, lb, ub);
+    has[d] = ok ? 1 : 0;
+}
+
+// Final pass without static variables: every cuboid is one critical leaf.
+template<class P>
+__global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList out, DeviceCounters* counters) {
+    const unsigned long long n = *fin.count;
+    const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool v = i < n;
+    double lb[P::N], ub[P::N];
+    if(v) load_box<P>(fin, i, lb, ub);
+    const unsigned long long pos = wave_append(v, out.count);
+    if(v && pos < out.cap) store_box<P>(out, pos, lb, ub);
+    wave_add(v ? 1ull : 0ull, &counters->leaves);
+    wave_add(v ? 1ull : 0ull, &counters->criticals);
+}
+
+// Initial runtime box: outward compile-time box, then 50*N rounds of all bounds.
+template<class P>
+__global__ void k_initial_box(const double* box_lb, const double* box_ub, const double* consts, double* out) {
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    load_consts<P>(s_c, consts);
+    enter_round_down();
+    if(threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lb[P::N], ub[P::N];
+#pragma unroll
+    for(int a = 0; a < P::N; ++a) { lb[a] = box_lb[a]; ub[a] = box_ub[a]; }
+    for(int it = 0; it < 50 * P::N; ++it) {
+        for(int k = 0; k < P::NB; ++k) {
+            if(P::apply(k, lb, ub, s_c) < 0) break;
+        }
+    }
+#pragma unroll
+    for(int a = 0; a < P::N; ++a) { out[a] = lb[a]; out[P::N + a] = ub[a]; }
+}
+
+} // namespace dev
+} // namespace sqd

@@ -1,0 +1,196 @@
+// Interval operations of the GPU prover core (binary64, outward rounded).
+//
+// Semantics follow the reference's IDouble operations bit for bit:
+//   + -            ivarp/include/ivarp/number/float_interval_addsub.hpp:53-102
+//   *              float_interval_mul.hpp:33-166 (sign cases, NaN -> 0 unless finite)
+//   /              float_interval_div.hpp:45-107, interval_div.hpp:57-288
+//   sqrt           math_fn/interval_sqrt.hpp:99-116 (clamp + undefined flag)
+//   square         math_fn/fixed_pow.hpp:183-187
+//   min / max      number/minmax.hpp:59-87
+//   comparisons    number/interval_comparisons.hpp:32-176, IBool bool.hpp:70-149
+//   if_then_else   math_fn/eval/if_then_else.hpp:64-82
+// Decisions the reference takes from compile-time bounds (sign of an operand,
+// finiteness) arrive here as template arguments: Tri::T / Tri::F when the
+// compile-time bounds decide, Tri::R when the runtime value decides.
+#pragma once
+#include "rounding.hpp"
+
+namespace sqd {
+namespace dev {
+
+struct IV { double lb, ub; bool und; };
+struct IB { bool def, poss; };
+
+enum Tri : int { F = 0, T = 1, R = 2 };
+
+__device__ __forceinline__ IV iv_add(IV a, IV b) { return IV{rd_add(a.lb, b.lb), ru_add(a.ub, b.ub), a.und || b.und}; }
+__device__ __forceinline__ IV iv_sub(IV a, IV b) { return IV{rd_sub(a.lb, b.ub), ru_sub(a.ub, b.lb), a.und || b.und}; }
+__device__ __forceinline__ IV iv_neg(IV a) { return IV{-a.ub, -a.lb, a.und}; }
+
+__device__ __forceinline__ double nan_to_zero(double v) { return v != v ? 0.0 : v; }
+
+template<bool Finite>
+__device__ __forceinline__ void do_mul(double l1, double l2, double u1, double u2, double& lo, double& hi) {
+    lo = rd_mul(l1, l2);
+    hi = ru_mul(u1, u2);
+    if(!Finite) { lo = nan_to_zero(lo); hi = nan_to_zero(hi); }
+}
+
+template<int C> __device__ __forceinline__ bool tri(bool runtime) { return C == T ? true : (C == F ? false : runtime); }
+
+// Multiplication with the reference's nine sign cases.  NN/NP: "non-negative" /
+// "non-positive" decisions for each operand.
+template<int NN1, int NP1, int NN2, int NP2, bool Finite>
+__device__ inline IV iv_mul(IV a, IV b) {
+    double lo, hi;
+    const bool a_nn = tri<NN1>(a.lb >= 0.0), a_np = tri<NP1>(a.ub <= 0.0);
+    const bool b_nn = tri<NN2>(b.lb >= 0.0), b_np = tri<NP2>(b.ub <= 0.0);
+    if(a_nn) {
+        if(b_nn) do_mul<Finite>(a.lb, b.lb, a.ub, b.ub, lo, hi);
+        else if(b_np) do_mul<Finite>(a.ub, b.lb, a.lb, b.ub, lo, hi);
+        else do_mul<Finite>(a.ub, b.lb, a.ub, b.ub, lo, hi);
+    } else if(a_np) {
+        if(b_nn) do_mul<Finite>(a.lb, b.ub, a.ub, b.lb, lo, hi);
+        else if(b_np) do_mul<Finite>(a.ub, b.ub, a.lb, b.lb, lo, hi);
+        else do_mul<Finite>(a.lb, b.ub, a.lb, b.lb, lo, hi);
+    } else if(b_nn) {
+        do_mul<Finite>(a.lb, b.ub, a.ub, b.ub, lo, hi);
+    } else if(b_np) {
+        do_mul<Finite>(a.ub, b.lb, a.lb, b.lb, lo, hi);
+    } else {
+        double l1, h1, l2, h2;
+        do_mul<Finite>(a.lb, b.ub, a.lb, b.lb, l1, h1);
+        do_mul<Finite>(b.lb, a.ub, a.ub, b.ub, l2, h2);
+        lo = (l2 <= l1) ? l2 : l1;
+        hi = (h1 <= h2) ? h2 : h1;
+    }
+    return IV{lo, hi, a.und || b.und};
+}
+
+// square(x) = x * x with run-time sign checks, then bounded below by 0.
+__device__ inline IV iv_square(IV a) {
+    IV r = iv_mul<R, R, R, R, false>(a, a);
+    if(r.lb < 0.0) r.lb = 0.0;
+    return r;
+}
+
+__device__ __forceinline__ int sgn(double v) { return (v > 0.0) - (v < 0.0); }
+
+__device__ inline void divide_by_zero(IV& num, IV den, int sl, int su, int dsum) {
+    const double inf = __builtin_huge_val();
+    num.und = true;
+    if(dsum == 0 || su - sl == 2) { num.lb = -inf; num.ub = inf; return; }
+    if(dsum > 0) {
+        if(sl >= 0) { num.lb = rd_div(num.lb, den.ub); num.ub = inf; }
+        else { num.ub = ru_div(num.ub, den.ub); num.lb = -inf; }
+    } else {
+        if(sl >= 0) { num.ub = ru_div(num.lb, den.lb); num.lb = -inf; }
+        else { num.lb = rd_div(num.ub, den.lb); num.ub = inf; }
+    }
+}
+
+__device__ __forceinline__ double div_lb_inf(double n, int ni, double d, int di) {
+    if(!ni && !di) return rd_div(n, d);
+    if(di != 0) return 0.0;
+    return -__builtin_huge_val();
+}
+__device__ __forceinline__ double div_ub_inf(double n, int ni, double d, int di) {
+    if(!ni && !di) return ru_div(n, d);
+    if(di != 0) return 0.0;
+    return __builtin_huge_val();
+}
+
+__device__ inline void divide_by_infinite(IV& num, IV den) {
+    num.und = num.und || den.und;
+    const int snl = sgn(num.lb), snu = sgn(num.ub), sdl = sgn(den.lb), sdu = sgn(den.ub);
+    const int inl = is_finite(num.lb) ? 0 : -1, inu = is_finite(num.ub) ? 0 : 1;
+    const int idl = is_finite(den.lb) ? 0 : -1, idu = is_finite(den.ub) ? 0 : 1;
+    double l, u;
+    if(sdl > 0) {
+        if(snl >= 0) { l = div_lb_inf(num.lb, inl, den.ub, idu); u = div_ub_inf(num.ub, inu, den.lb, idl); }
+        else if(snu <= 0) { l = div_lb_inf(num.lb, inl, den.lb, idl); u = div_ub_inf(num.ub, inu, den.ub, idu); }
+        else { l = div_lb_inf(num.lb, inl, den.lb, idl); u = div_ub_inf(num.ub, inu, den.lb, idl); }
+    } else if(sdu < 0) {
+        if(snl >= 0) { l = div_lb_inf(num.ub, inu, den.ub, idu); u = div_ub_inf(num.lb, inl, den.lb, idl); }
+        else if(snu <= 0) { l = div_lb_inf(num.ub, inu, den.lb, idl); u = div_ub_inf(num.lb, inl, den.ub, idu); }
+        else { l = div_lb_inf(num.ub, inu, den.ub, idu); u = div_ub_inf(num.lb, inl, den.ub, idu); }
+    } else {
+        divide_by_zero(num, den, snl, snu, sdl + sdu);
+        return;
+    }
+    num.lb = l;
+    num.ub = u;
+}
+
+// Division; POS/NEG: "denominator positive/negative"; NN/NP for the numerator;
+// CTFinite: both operands have finite compile-time bounds.
+template<int NN, int NP, int POS, int NEG, bool CTFinite>
+__device__ inline IV iv_div(IV num, IV den) {
+    if(!CTFinite && !(is_finite(num.lb) && is_finite(num.ub) && is_finite(den.lb) && is_finite(den.ub))) {
+        divide_by_infinite(num, den);
+        return num;
+    }
+    num.und = num.und || den.und;
+    double l, u;
+    if(tri<POS>(den.lb > 0.0)) {
+        if(tri<NN>(num.lb >= 0.0)) { l = rd_div(num.lb, den.ub); u = ru_div(num.ub, den.lb); }
+        else if(tri<NP>(num.ub <= 0.0)) { l = rd_div(num.lb, den.lb); u = ru_div(num.ub, den.ub); }
+        else { l = rd_div(num.lb, den.lb); u = ru_div(num.ub, den.lb); }
+    } else if(tri<NEG>(den.ub < 0.0)) {
+        if(tri<NN>(num.lb >= 0.0)) { l = rd_div(num.ub, den.ub); u = ru_div(num.lb, den.lb); }
+        else if(tri<NP>(num.ub <= 0.0)) { l = rd_div(num.ub, den.lb); u = ru_div(num.lb, den.ub); }
+        else { l = rd_div(num.ub, den.ub); u = ru_div(num.lb, den.ub); }
+    } else {
+        divide_by_zero(num, den, sgn(num.lb), sgn(num.ub), sgn(den.lb) + sgn(den.ub));
+        return num;
+    }
+    num.lb = l;
+    num.ub = u;
+    return num;
+}
+
+template<int NN>
+__device__ inline IV iv_sqrt(IV x) {
+    double lb = x.lb, ub = x.ub;
+    bool und = x.und;
+    if(!tri<NN>(x.lb >= 0.0)) {
+        if(ub < 0.0) return IV{-__builtin_huge_val(), __builtin_huge_val(), true};
+        lb = 0.0;
+        und = true;
+    }
+    return IV{rd_sqrt(lb), ru_sqrt(ub), und};
+}
+
+__device__ __forceinline__ double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }
+__device__ __forceinline__ double dmin(double n1, double n2) { return n1 < n2 ? n1 : n2; }
+__device__ __forceinline__ IV iv_max(IV a, IV b) { return IV{dmax(a.lb, b.lb), dmax(a.ub, b.ub), a.und || b.und}; }
+__device__ __forceinline__ IV iv_min(IV a, IV b) { return IV{dmin(a.lb, b.lb), dmin(a.ub, b.ub), a.und || b.und}; }
+
+__device__ __forceinline__ IV iv_ite(IB c, IV t, IV e) {
+    if(c.def) return t;
+    if(!c.poss) return e;
+    IV r = t;
+    if(e.lb < r.lb) r.lb = e.lb;
+    if(e.ub > r.ub) r.ub = e.ub;
+    r.und = r.und || e.und;
+    return r;
+}
+
+__device__ __forceinline__ IB ib_less(IV a, IV b) {
+    return IB{!a.und && !b.und && a.ub < b.lb, a.und || b.und || a.lb < b.ub};
+}
+__device__ __forceinline__ IB ib_not(IB x) { return IB{!x.poss, !x.def}; }
+__device__ __forceinline__ IB ib_lt(IV a, IV b) { return ib_less(a, b); }
+__device__ __forceinline__ IB ib_gt(IV a, IV b) { return ib_less(b, a); }
+__device__ __forceinline__ IB ib_ge(IV a, IV b) { return ib_not(ib_less(a, b)); }
+__device__ __forceinline__ IB ib_le(IV a, IV b) { return ib_not(ib_less(b, a)); }
+__device__ __forceinline__ IB ib_eq(IV a, IV b) {
+    return IB{!a.und && !b.und && a.lb == a.ub && b.lb == b.ub && a.lb == b.lb,
+              a.und || b.und || (!(a.ub < b.lb) && !(b.ub < a.lb))};
+}
+__device__ __forceinline__ IB ib_neq(IV a, IV b) { return ib_not(ib_eq(a, b)); }
+__device__ __forceinline__ IB ib_or(IB a, IB b) { return IB{a.def || b.def, a.poss || b.poss}; }
+__device__ __forceinline__ IB ib_and(IB a, IB b) { return IB{a.def && b.def, a.poss && b.poss}; }
+
+} // namespace dev
+} // namespace sqd

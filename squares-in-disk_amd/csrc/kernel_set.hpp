@@ -1,0 +1,123 @@
+// Host-side launch table for the kernels of one compiled program.  Each
+// generated program file instantiates make_kernel_set<P>() and registers it under
+// the program's fingerprint; the runtime looks the table up when a program is
+// bound (sqd_program_from_text / sqd_program_from_lemma).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <utility>
+#include "core.hpp"
+
+namespace sqd {
+
+struct KernelSet {
+    std::uint64_t fingerprint;
+    const char* key;
+    int N, NB, NC, NDYN, NSTAT;
+    const double* consts;   // 2*NC doubles: [lb, ub] per constant
+    hipError_t (*static_level)(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s);
+    hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s);
+    hipError_t (*hull_finalize)(dev::BoxList items, unsigned long long base, unsigned long long n,
+                                const unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
+                                dev::DeviceCounters* c, unsigned gens, hipStream_t s);
+    hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s);
+    hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out, hipStream_t s);
+    hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, const unsigned long long* hull,
+                              dev::BoxList out, unsigned char* has, hipStream_t s);
+};
+
+void register_kernel_set(const KernelSet* ks);
+const KernelSet* find_kernel_set(std::uint64_t fingerprint);
+
+namespace detail {
+
+template<class P, int MODE, int L>
+hipError_t launch_one_static(const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L>), dim3(grid), dim3(dev::kBlock), 0, s, a);
+    return hipGetLastError();
+}
+template<class P, int MODE, int J>
+hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_dynamic_level<P, MODE, J>), dim3(grid), dim3(dev::kBlock), 0, s, a);
+    return hipGetLastError();
+}
+template<class P, int MODE, int... L>
+hipError_t launch_static(int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s, std::integer_sequence<int, L...>) {
+    hipError_t e = hipErrorInvalidValue;
+    (void)a; (void)grid; (void)s; (void)level;
+    ((level == L ? (e = launch_one_static<P, MODE, L>(a, grid, s), 0) : 0), ...);
+    return e;
+}
+template<class P, int MODE, int... J>
+hipError_t launch_dynamic(int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s, std::integer_sequence<int, J...>) {
+    hipError_t e = hipErrorInvalidValue;
+    (void)a; (void)grid; (void)s; (void)level;
+    ((level == J ? (e = launch_one_dynamic<P, MODE, J>(a, grid, s), 0) : 0), ...);
+    return e;
+}
+
+template<class P>
+hipError_t static_level(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+    auto seq = std::make_integer_sequence<int, P::NSTAT>{};
+    if(mode == dev::NONFINAL) return launch_static<P, dev::NONFINAL>(level, a, grid, s, seq);
+    if(mode == dev::FINAL) return launch_static<P, dev::FINAL>(level, a, grid, s, seq);
+    return hipErrorInvalidValue;
+}
+template<class P>
+hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+    auto seq = std::make_integer_sequence<int, P::NDYN>{};
+    if(mode == dev::SPLIT) return launch_dynamic<P, dev::SPLIT>(level, a, grid, s, seq);
+    if(mode == dev::INIT) return launch_dynamic<P, dev::INIT>(level, a, grid, s, seq);
+    return hipErrorInvalidValue;
+}
+template<class P>
+hipError_t hull_finalize(dev::BoxList items, unsigned long long base, unsigned long long n, const unsigned long long* hull,
+                         dev::BoxList fin, dev::BoxList spl, dev::DeviceCounters* c, unsigned gens, hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);
+    hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, base, n, hull, fin, spl, c, gens);
+    return hipGetLastError();
+}
+template<class P>
+hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_final_nostatic<P>), dim3(grid), dim3(dev::kBlock), 0, s, fin, out, c);
+    return hipGetLastError();
+}
+template<class P>
+hipError_t initial_box(const double* lb, const double* ub, const double* consts, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out);
+    return hipGetLastError();
+}
+
+template<class P>
+hipError_t hull_decode(dev::BoxList items, unsigned long long n, const unsigned long long* hull, dev::BoxList out,
+                       unsigned char* has, hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);
+    hipLaunchKernelGGL((dev::k_hull_decode<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, n, hull, out, has);
+    return hipGetLastError();
+}
+
+} // namespace detail
+
+template<class P>
+KernelSet make_kernel_set(std::uint64_t fingerprint, const char* key, const double* consts) {
+    KernelSet k;
+    k.fingerprint = fingerprint;
+    k.key = key;
+    k.N = P::N;
+    k.NB = P::NB;
+    k.NC = P::NC;
+    k.NDYN = P::NDYN;
+    k.NSTAT = P::NSTAT;
+    k.consts = consts;
+    k.static_level = &detail::static_level<P>;
+    k.dynamic_level = &detail::dynamic_level<P>;
+    k.hull_finalize = &detail::hull_finalize<P>;
+    k.final_nostatic = &detail::final_nostatic<P>;
+    k.initial_box = &detail::initial_box<P>;
+    k.hull_decode = &detail::hull_decode<P>;
+    return k;
+}
+
+} // namespace sqd

@@ -1,0 +1,583 @@
+// Host runtime of the GPU prover core: program binding, device buffers and the
+// round loop of the proof driver, exported through the C ABI of include/sqd.h.
+//
+// The proof runs as rounds over a LIFO frontier resident in HBM
+// (prover/proof_driver.hpp:393-438 pops up to dequeue_buffer_size cuboids per
+// batch from a LIFO queue): each round pops K dynamic cuboids from the top of
+// the device stack, runs the static levels (non-final pass, hull per cuboid),
+// sorts the hulls into final (depth >= generation_count) and split lists, runs
+// the final pass and the dynamic split, and pushes the children back onto the
+// stack.  The count totals do not depend on the processing order.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sqd.h"
+#include "kernel_set.hpp"
+#include "ivarp/program.hpp"
+#include "lemmas.hpp"
+
+namespace sqd {
+
+static std::vector<const KernelSet*>& registry() {
+    static std::vector<const KernelSet*> r;
+    return r;
+}
+void register_kernel_set(const KernelSet* ks) { registry().push_back(ks); }
+const KernelSet* find_kernel_set(std::uint64_t fp) {
+    for(const KernelSet* k : registry()) if(k->fingerprint == fp) return k;
+    return nullptr;
+}
+
+namespace {
+
+thread_local std::string t_error;
+
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if(e != hipSuccess) throw Failure(SQD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template<typename F> int guarded(F&& f) {
+    try { f(); return SQD_OK; }
+    catch(const Failure& e) { t_error = e.what(); return e.code; }
+    catch(const std::invalid_argument& e) { t_error = e.what(); return SQD_ERR_ARGUMENT; }
+    catch(const std::exception& e) { t_error = e.what(); return SQD_ERR_INTERNAL; }
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+    if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
+    for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+        i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) p[i] = v;
+}
+// hull record initial values: min-key, max-key, zero-sign lb, zero-sign ub
+__global__ void k_init_hull(unsigned long long* p, unsigned long long n_records) {
+    for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_records;
+        i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        p[4 * i + 0] = ~0ull;
+        p[4 * i + 1] = 0ull;
+        p[4 * i + 2] = ~0ull;
+        p[4 * i + 3] = ~0ull;
+    }
+}
+
+template<typename T> T* dalloc(std::size_t n) {
+    void* p = nullptr;
+    if(n == 0) n = 1;
+    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+// A device box list with its own storage.
+struct DevList {
+    int N = 0;
+    unsigned long long cap = 0;
+    double* lb = nullptr;
+    double* ub = nullptr;
+    unsigned* depth = nullptr;
+    unsigned* origin = nullptr;
+    unsigned* leaf = nullptr;
+    unsigned long long* count = nullptr;
+    void create(int n_args, unsigned long long capacity, bool with_depth, bool with_origin) {
+        N = n_args;
+        cap = capacity;
+        lb = dalloc<double>(static_cast<std::size_t>(cap) * N);
+        ub = dalloc<double>(static_cast<std::size_t>(cap) * N);
+        if(with_depth) depth = dalloc<unsigned>(cap);
+        if(with_origin) { origin = dalloc<unsigned>(cap); leaf = dalloc<unsigned>(cap); }
+        count = dalloc<unsigned long long>(1);
+    }
+    void destroy() {
+        for(void* p : {static_cast<void*>(lb), static_cast<void*>(ub), static_cast<void*>(depth),
+                       static_cast<void*>(origin), static_cast<void*>(leaf), static_cast<void*>(count)})
+            if(p) (void)hipFree(p);
+        lb = ub = nullptr; depth = origin = leaf = nullptr; count = nullptr;
+    }
+    dev::BoxList view() const {
+        dev::BoxList v;
+        v.lb = lb; v.ub = ub; v.depth = depth; v.origin = origin; v.leaf = leaf;
+        v.stride = cap; v.cap = cap; v.count = count;
+        return v;
+    }
+};
+
+std::uint64_t product(const std::vector<std::uint32_t>& v, std::size_t upto) {
+    std::uint64_t p = 1;
+    for(std::size_t i = 0; i < upto && i < v.size(); ++i) p *= v[i];
+    return p;
+}
+
+} // namespace
+} // namespace sqd
+
+struct sqd_program {
+    ivarp::flat::Program prog;
+    const sqd::KernelSet* ks = nullptr;
+    std::string text;
+};
+
+struct sqd_core {
+    int device = 0;
+    const sqd_program* program = nullptr;
+    const sqd::KernelSet* ks = nullptr;
+    sqd_settings settings{};
+    hipStream_t stream = nullptr;
+    int N = 0;
+    unsigned grid = 0;
+    double* d_consts = nullptr;
+    double* d_box_ct = nullptr;     // 2N: outward compile-time box
+    double* d_box_init = nullptr;   // 2N: initial runtime box
+    sqd::dev::DeviceCounters* d_counters = nullptr;
+    unsigned long long* d_hull = nullptr;
+    unsigned char* d_has = nullptr;
+    sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
+    unsigned long long k_max = 0;    // dynamic cuboids per round
+    unsigned long long split_fanout = 1;
+    std::vector<std::uint32_t> st_subs, dyn_subs, dyn_initial;
+};
+
+namespace sqd {
+namespace {
+
+void set_count(sqd_core* c, unsigned long long* p, unsigned long long v) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c->stream, p, v);
+    hip_check(hipGetLastError(), "k_set_u64");
+}
+unsigned long long get_count(sqd_core* c, unsigned long long* p) {
+    unsigned long long v = 0;
+    hip_check(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync count");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return v;
+}
+void reset_counters(sqd_core* c) {
+    hip_check(hipMemsetAsync(c->d_counters, 0, sizeof(dev::DeviceCounters), c->stream), "hipMemsetAsync counters");
+}
+sqd_counts read_counters(sqd_core* c) {
+    dev::DeviceCounters h{};
+    hip_check(hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy counters");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if(h.overflow) throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
+    sqd_counts r;
+    r.num_cuboids = h.cuboids;
+    r.num_leaf_cuboids = h.leaves;
+    r.num_critical_cuboids = h.criticals;
+    r.num_repeated_nodes = 0;
+    return r;
+}
+void add_counts(sqd_counts* out, const sqd_counts& c) {
+    if(!out) return;
+    out->num_cuboids += c.num_cuboids;
+    out->num_leaf_cuboids += c.num_leaf_cuboids;
+    out->num_critical_cuboids += c.num_critical_cuboids;
+    out->num_repeated_nodes += c.num_repeated_nodes;
+}
+
+dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, unsigned long long base, unsigned long long fixed,
+                          const dev::BoxList& out) {
+    dev::LevelArgs a;
+    a.in = in;
+    a.out = out;
+    a.in_base = base;
+    a.in_fixed_count = fixed;
+    a.hull = c->d_hull;
+    a.counters = c->d_counters;
+    a.consts = c->d_consts;
+    a.generations = c->settings.generation_count;
+    return a;
+}
+
+// Static levels over `n` inputs taken from `in` (fixed count when n > 0, else
+// the list's device counter).  NONFINAL joins the leaves into d_hull;
+// FINAL appends critical leaves to `crit_out`.
+void run_static(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long base, unsigned long long n,
+                const dev::BoxList& crit_out) {
+    const int S = c->ks->NSTAT;
+    dev::BoxList prev = in;
+    unsigned long long prev_base = base, prev_fixed = n;
+    for(int L = 0; L < S; ++L) {
+        const bool last = (L == S - 1);
+        dev::BoxList out = last ? crit_out : c->lvl[L % 2].view();
+        if(!last) set_count(c, out.count, 0);
+        dev::BoxList src = prev;
+        if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
+        dev::LevelArgs a = level_args(c, src, prev_base, prev_fixed, out);
+        hip_check(c->ks->static_level(mode, L, a, c->grid, c->stream), "static level launch");
+        prev = c->lvl[L % 2].view();
+        prev_base = 0;
+        prev_fixed = 0;
+    }
+}
+
+// Dynamic levels (SPLIT or INIT) from `in` to `out` (appending to its counter).
+void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long n, const dev::BoxList& out) {
+    const int D = c->ks->NDYN;
+    dev::BoxList prev = in;
+    unsigned long long prev_fixed = n;
+    for(int J = 0; J < D; ++J) {
+        const bool last = (J == D - 1);
+        dev::BoxList o = last ? out : c->lvl[J % 2].view();
+        if(!last) set_count(c, o.count, 0);
+        o.origin = nullptr;
+        o.leaf = nullptr;
+        if(!last) o.depth = c->lvl[J % 2].depth;
+        dev::BoxList src = prev;
+        src.origin = nullptr;
+        src.leaf = nullptr;
+        dev::LevelArgs a = level_args(c, src, 0, prev_fixed, o);
+        hip_check(c->ks->dynamic_level(mode, J, a, c->grid, c->stream), "dynamic level launch");
+        prev = c->lvl[J % 2].view();
+        prev_fixed = 0;
+    }
+}
+
+void upload_soa(sqd_core* c, DevList& l, const double* lb, const double* ub, unsigned long long n, unsigned depth) {
+    if(n > l.cap) throw Failure(SQD_ERR_CAPACITY, "input larger than the device list");
+    for(int a = 0; a < c->N; ++a) {
+        hip_check(hipMemcpyAsync(l.lb + static_cast<std::size_t>(a) * l.cap, lb + static_cast<std::size_t>(a) * n,
+                                 n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload lb");
+        hip_check(hipMemcpyAsync(l.ub + static_cast<std::size_t>(a) * l.cap, ub + static_cast<std::size_t>(a) * n,
+                                 n * sizeof(double), hipMemcpyHostToDevice, c->stream), "upload ub");
+    }
+    std::vector<unsigned> d;
+    if(l.depth) {
+        d.assign(n, depth);
+        hip_check(hipMemcpyAsync(l.depth, d.data(), n * sizeof(unsigned), hipMemcpyHostToDevice, c->stream), "upload depth");
+    }
+    set_count(c, l.count, n);
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* lb, double* ub, unsigned long long stride) {
+    for(int a = 0; a < c->N; ++a) {
+        hip_check(hipMemcpyAsync(lb + static_cast<std::size_t>(a) * stride, l.lb + static_cast<std::size_t>(a) * l.cap,
+                                 n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download lb");
+        hip_check(hipMemcpyAsync(ub + static_cast<std::size_t>(a) * stride, l.ub + static_cast<std::size_t>(a) * l.cap,
+                                 n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download ub");
+    }
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+void compute_initial_box(sqd_core* c) {
+    hip_check(c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, c->stream), "initial box");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+// Initial queue onto the (empty) stack; returns its length.
+unsigned long long build_initial_queue(sqd_core* c) {
+    compute_initial_box(c);
+    std::vector<double> box(2 * static_cast<std::size_t>(c->N));
+    hip_check(hipMemcpy(box.data(), c->d_box_init, box.size() * sizeof(double), hipMemcpyDeviceToHost), "copy box");
+    upload_soa(c, c->scratch, box.data(), box.data() + c->N, 1, 0);
+    set_count(c, c->stack.count, 0);
+    run_dynamic(c, dev::INIT, c->scratch.view(), 1, c->stack.view());
+    return get_count(c, c->stack.count);
+}
+
+// One round over the top K cuboids of the stack; returns the new stack size.
+unsigned long long run_round(sqd_core* c, unsigned long long S, unsigned long long K) {
+    const unsigned long long base = S - K;
+    const int N = c->N;
+    dev::BoxList items = c->stack.view();
+    if(c->ks->NSTAT > 0) {
+        hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, K * static_cast<unsigned long long>(N));
+        hip_check(hipGetLastError(), "k_init_hull");
+        run_static(c, dev::NONFINAL, items, base, K, c->crit.view());
+    }
+    set_count(c, c->fin.count, 0);
+    set_count(c, c->spl.count, 0);
+    hip_check(c->ks->hull_finalize(items, base, K, c->d_hull, c->fin.view(), c->spl.view(), c->d_counters,
+                                   c->settings.generation_count, c->stream), "hull finalize");
+    set_count(c, c->stack.count, base);
+    // final pass over hulls at the last generation
+    if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
+    else hip_check(c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
+                                         static_cast<unsigned>((K + 255) / 256), c->stream), "final");
+    // dynamic split of the other hulls, children pushed onto the stack
+    dev::BoxList spl = c->spl.view();
+    run_dynamic(c, dev::SPLIT, spl, 0, c->stack.view());
+    return get_count(c, c->stack.count);
+}
+
+} // namespace
+} // namespace sqd
+
+using namespace sqd;
+
+extern "C" {
+
+const char* sqd_last_error(void) { return t_error.c_str(); }
+
+void sqd_settings_default(sqd_settings* s) {
+    if(!s) return;
+    *s = sqd_settings{};
+    s->generation_count = 8;
+}
+
+int sqd_lemma_count(void) { return static_cast<int>(sqd_lemmas::all_lemmas().size()); }
+const char* sqd_lemma_key(int i) {
+    const auto& l = sqd_lemmas::all_lemmas();
+    return (i >= 0 && static_cast<std::size_t>(i) < l.size()) ? l[static_cast<std::size_t>(i)].key.c_str() : nullptr;
+}
+const char* sqd_lemma_title(int i) {
+    const auto& l = sqd_lemmas::all_lemmas();
+    return (i >= 0 && static_cast<std::size_t>(i) < l.size()) ? l[static_cast<std::size_t>(i)].title.c_str() : nullptr;
+}
+
+static void bind(sqd_program* p) {
+    std::ostringstream o;
+    p->prog.write(o);
+    p->text = o.str();
+    p->ks = find_kernel_set(p->prog.fingerprint());
+}
+
+int sqd_program_from_text(const char* text, sqd_program** out) {
+    return guarded([&] {
+        if(!text || !out) throw std::invalid_argument("null argument");
+        std::istringstream in(text);
+        auto p = std::make_unique<sqd_program>();
+        try { p->prog = ivarp::flat::Program::read(in); }
+        catch(const std::runtime_error& e) { throw std::invalid_argument(e.what()); }
+        bind(p.get());
+        *out = p.release();
+    });
+}
+
+int sqd_program_from_lemma(const char* key, sqd_program** out) {
+    return guarded([&] {
+        if(!key || !out) throw std::invalid_argument("null argument");
+        const auto& d = sqd_lemmas::lemma(key);
+        auto p = std::make_unique<sqd_program>();
+        p->prog = sqd_lemmas::lower_lemma(d);
+        bind(p.get());
+        *out = p.release();
+    });
+}
+
+void sqd_program_destroy(sqd_program* p) { delete p; }
+
+int sqd_program_text(const sqd_program* p, char* buffer, size_t capacity, size_t* len) {
+    return guarded([&] {
+        if(!p) throw std::invalid_argument("null program");
+        if(len) *len = p->text.size() + 1;
+        if(buffer && capacity > 0) {
+            std::size_t n = std::min(capacity - 1, p->text.size());
+            std::memcpy(buffer, p->text.data(), n);
+            buffer[n] = 0;
+        }
+    });
+}
+int sqd_program_nargs(const sqd_program* p) { return p ? p->prog.nargs : -1; }
+int sqd_program_has_kernels(const sqd_program* p) { return (p && p->ks) ? 1 : 0; }
+
+int sqd_core_create(int device, const sqd_program* program, const sqd_settings* settings, sqd_core** out) {
+    return guarded([&] {
+        if(!program || !out) throw std::invalid_argument("null argument");
+        if(!program->ks)
+            throw Failure(SQD_ERR_NO_KERNELS, "no kernels are compiled for this program (fingerprint not registered)");
+        auto c = std::make_unique<sqd_core>();
+        c->device = device;
+        c->program = program;
+        c->ks = program->ks;
+        c->N = program->prog.nargs;
+        if(settings) c->settings = *settings;
+        else sqd_settings_default(&c->settings);
+        sqd_settings& s = c->settings;
+        if(s.frontier_capacity == 0) s.frontier_capacity = 1ull << 23;
+        if(s.level_bytes == 0) s.level_bytes = 1ull << 30;
+        if(s.critical_capacity == 0) s.critical_capacity = 1ull << 20;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        c->grid = s.grid_blocks ? s.grid_blocks : static_cast<unsigned>(prop.multiProcessorCount) * 8u;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        const auto& prog = program->prog;
+        for(const auto& sp : prog.splits) {
+            if(sp.dynamic) { c->dyn_subs.push_back(sp.subs); c->dyn_initial.push_back(sp.initial); }
+            else c->st_subs.push_back(sp.subs);
+        }
+        // level buffers hold the survivors of one level; size K for the worst case
+        const unsigned long long per_entry = 16ull * static_cast<unsigned long long>(c->N) + 12ull;
+        const unsigned long long lvl_cap = std::max<unsigned long long>(1024, s.level_bytes / per_entry);
+        unsigned long long worst_static = 1, worst_split = 1;
+        for(std::size_t L = 0; L + 1 < c->st_subs.size(); ++L) worst_static = std::max<unsigned long long>(worst_static, product(c->st_subs, L + 1));
+        for(std::size_t J = 0; J + 1 < c->dyn_subs.size(); ++J) worst_split = std::max<unsigned long long>(worst_split, product(c->dyn_subs, J + 1));
+        unsigned long long worst_init = product(c->dyn_initial, c->dyn_initial.size() > 0 ? c->dyn_initial.size() - 1 : 0);
+        if(worst_init > lvl_cap) throw Failure(SQD_ERR_CAPACITY, "initial queue levels do not fit the level buffers");
+        c->split_fanout = product(c->dyn_subs, c->dyn_subs.size());
+        unsigned long long kmax = s.dequeue_buffer_size ? s.dequeue_buffer_size : 8192;
+        kmax = std::min(kmax, lvl_cap / worst_static);
+        kmax = std::min(kmax, lvl_cap / worst_split);
+        if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
+        c->k_max = kmax;
+        c->stack.create(c->N, std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1), true, false);
+        for(auto& l : c->lvl) l.create(c->N, lvl_cap, true, true);
+        c->fin.create(c->N, kmax, true, false);
+        c->spl.create(c->N, kmax, true, false);
+        c->crit.create(c->N, s.critical_capacity, false, false);
+        c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false);
+        c->d_hull = dalloc<unsigned long long>(static_cast<std::size_t>(kmax) * c->N * 4);
+        c->d_has = dalloc<unsigned char>(static_cast<std::size_t>(kmax));
+        c->d_counters = dalloc<dev::DeviceCounters>(1);
+        c->d_consts = dalloc<double>(2 * static_cast<std::size_t>(std::max(1, c->ks->NC)));
+        hip_check(hipMemcpy(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
+                            hipMemcpyHostToDevice), "upload constants");
+        c->d_box_ct = dalloc<double>(2 * static_cast<std::size_t>(c->N));
+        c->d_box_init = dalloc<double>(2 * static_cast<std::size_t>(c->N));
+        std::vector<double> box(prog.box_lb);
+        box.insert(box.end(), prog.box_ub.begin(), prog.box_ub.end());
+        hip_check(hipMemcpy(c->d_box_ct, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice), "upload box");
+        *out = c.release();
+    });
+}
+
+void sqd_core_destroy(sqd_core* c) {
+    if(!c) return;
+    (void)hipSetDevice(c->device);
+    if(c->stream) (void)hipStreamSynchronize(c->stream);
+    c->stack.destroy();
+    for(auto& l : c->lvl) l.destroy();
+    c->fin.destroy(); c->spl.destroy(); c->crit.destroy(); c->scratch.destroy();
+    for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_counters),
+                   static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
+        if(p) (void)hipFree(p);
+    if(c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
+    return guarded([&] {
+        if(!c || !lb || !ub) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        compute_initial_box(c);
+        hip_check(hipMemcpy(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpy(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
+    });
+}
+
+int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size_t* n, sqd_counts* counts) {
+    return guarded([&] {
+        if(!c || !n) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        reset_counters(c);
+        unsigned long long len = build_initial_queue(c);
+        *n = len;
+        if(lb && ub) download_soa(c, c->stack, std::min<unsigned long long>(len, capacity), lb, ub, capacity);
+        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+    });
+}
+
+int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
+                 double* hull_lb, double* hull_ub, uint8_t* has_hull, sqd_counts* counts) {
+    return guarded([&] {
+        if(!c || (n && (!lb || !ub || !hull_lb || !hull_ub || !has_hull))) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        reset_counters(c);
+        std::vector<double> clb, cub, hlb, hub;
+        for(std::size_t off = 0; off < n; off += c->k_max) {
+            const std::size_t m = std::min<std::size_t>(c->k_max, n - off);
+            clb.assign(static_cast<std::size_t>(c->N) * m, 0);
+            cub.assign(static_cast<std::size_t>(c->N) * m, 0);
+            for(int a = 0; a < c->N; ++a)
+                for(std::size_t i = 0; i < m; ++i) {
+                    clb[static_cast<std::size_t>(a) * m + i] = lb[static_cast<std::size_t>(a) * n + off + i];
+                    cub[static_cast<std::size_t>(a) * m + i] = ub[static_cast<std::size_t>(a) * n + off + i];
+                }
+            upload_soa(c, c->stack, clb.data(), cub.data(), m, 0);
+            if(c->ks->NSTAT > 0) {
+                hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, m * static_cast<unsigned long long>(c->N));
+                hip_check(hipGetLastError(), "k_init_hull");
+                run_static(c, dev::NONFINAL, c->stack.view(), 0, m, c->crit.view());
+            }
+            hip_check(c->ks->hull_decode(c->stack.view(), m, c->d_hull, c->scratch.view(), c->d_has, c->stream), "decode");
+            hlb.assign(static_cast<std::size_t>(c->N) * m, 0);
+            hub.assign(static_cast<std::size_t>(c->N) * m, 0);
+            download_soa(c, c->scratch, m, hlb.data(), hub.data(), m);
+            hip_check(hipMemcpy(has_hull + off, c->d_has, m, hipMemcpyDeviceToHost), "copy has");
+            for(int a = 0; a < c->N; ++a)
+                for(std::size_t i = 0; i < m; ++i) {
+                    hull_lb[static_cast<std::size_t>(a) * n + off + i] = hlb[static_cast<std::size_t>(a) * m + i];
+                    hull_ub[static_cast<std::size_t>(a) * n + off + i] = hub[static_cast<std::size_t>(a) * m + i];
+                }
+        }
+        sqd_counts r = read_counters(c);
+        if(c->ks->NSTAT == 0) r.num_leaf_cuboids += n;  // without static variables each box is its own leaf
+        if(counts) *counts = r;
+    });
+}
+
+int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
+              double* crit_lb, double* crit_ub, size_t capacity, size_t* n_critical, sqd_counts* counts) {
+    return guarded([&] {
+        if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        reset_counters(c);
+        set_count(c, c->crit.count, 0);
+        if(n > c->fin.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one final pass");
+        upload_soa(c, c->fin, lb, ub, n, 0);
+        if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
+        else hip_check(c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
+                                             static_cast<unsigned>((n + 255) / 256 + 1), c->stream), "final");
+        unsigned long long k = get_count(c, c->crit.count);
+        if(n_critical) *n_critical = k;
+        if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+    });
+}
+
+int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
+              double* out_lb, double* out_ub, size_t capacity, size_t* n_out, sqd_counts* counts) {
+    return guarded([&] {
+        if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        reset_counters(c);
+        if(n > c->spl.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one split");
+        upload_soa(c, c->spl, lb, ub, n, 1);
+        set_count(c, c->stack.count, 0);
+        run_dynamic(c, dev::SPLIT, c->spl.view(), 0, c->stack.view());
+        unsigned long long k = get_count(c, c->stack.count);
+        if(n_out) *n_out = k;
+        if(out_lb && out_ub) download_soa(c, c->stack, std::min<unsigned long long>(k, capacity), out_lb, out_ub, capacity);
+        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+    });
+}
+
+int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
+              size_t capacity, size_t* n_critical, double* seconds) {
+    return guarded([&] {
+        if(!c) throw std::invalid_argument("null core");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        auto t0 = std::chrono::steady_clock::now();
+        reset_counters(c);
+        set_count(c, c->crit.count, 0);
+        unsigned long long S = build_initial_queue(c);
+        const unsigned long long cap = c->stack.cap;
+        while(S > 0) {
+            unsigned long long room = cap > S ? cap - S : 0;
+            unsigned long long K = std::min(c->k_max, S);
+            if(c->split_fanout > 1) K = std::min(K, room / (c->split_fanout - 1));
+            if(K == 0) throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
+            S = run_round(c, S, K);
+        }
+        sqd_counts r = read_counters(c);
+        auto t1 = std::chrono::steady_clock::now();
+        if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+        if(counts) *counts = r;
+        unsigned long long k = get_count(c, c->crit.count);
+        if(n_critical) *n_critical = k;
+        if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,231 @@
+// Emits the HIP program struct of one lowered program (see csrc/core.hpp for
+// the interface).  Compile-time decisions that the reference takes from its
+// fixed-point bounds (operand signs, finiteness, whether an undefined bound value
+// disables a bound) are resolved here and become template arguments or are
+// dropped from the generated code.
+#include "codegen.hpp"
+#include "fixed_point.hpp"
+#include <sstream>
+#include <stdexcept>
+#include <map>
+#include <cstdio>
+
+namespace ivarp {
+namespace codegen {
+namespace {
+using flat::FOp;
+using flat::Program;
+namespace fpb = fixed_point_bounds;
+
+std::string dlit(double d) {
+    if(d == __builtin_huge_val()) return "__builtin_huge_val()";
+    if(d == -__builtin_huge_val()) return "(-__builtin_huge_val())";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", d);
+    return buf;
+}
+int tri_nn(const flat::FNode& b) { return b.lb >= 0 ? 1 : (b.ub <= 0 ? 0 : 2); }
+int tri_np(const flat::FNode& b) { return b.ub <= 0 ? 1 : (b.lb >= 0 ? 0 : 2); }
+int tri_pos(const flat::FNode& b) { return b.lb > 0 ? 1 : (b.ub <= 0 ? 0 : 2); }
+int tri_neg(const flat::FNode& b) { return b.ub < 0 ? 1 : (b.lb >= 0 ? 0 : 2); }
+bool ct_finite(const flat::FNode& b) { return b.lb > fpb::min_bound() && b.ub < fpb::max_bound(); }
+
+struct Emitter {
+    const Program& p;
+    std::map<int, int> const_index;
+    std::vector<double> consts;
+
+    int cidx(int node) {
+        auto it = const_index.find(node);
+        if(it != const_index.end()) return it->second;
+        int i = static_cast<int>(consts.size() / 2);
+        consts.push_back(p.nodes[static_cast<std::size_t>(node)].c_lb);
+        consts.push_back(p.nodes[static_cast<std::size_t>(node)].c_ub);
+        const_index[node] = i;
+        return i;
+    }
+    void collect(int id, std::vector<int>& order, std::vector<char>& seen) {
+        if(seen[static_cast<std::size_t>(id)]) return;
+        seen[static_cast<std::size_t>(id)] = 1;
+        for(int k : p.nodes[static_cast<std::size_t>(id)].kids) collect(k, order, seen);
+        order.push_back(id);
+    }
+    std::string v(int id) { return "v" + std::to_string(id); }
+    std::string b(int id) { return "p" + std::to_string(id); }
+    // Straight-line evaluation of the sub-DAG below `root`.
+    std::string eval(int root, const std::string& indent) {
+        std::vector<int> order;
+        std::vector<char> seen(p.nodes.size(), 0);
+        collect(root, order, seen);
+        std::ostringstream o;
+        for(int id : order) {
+            const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
+            auto K = [&](std::size_t i) { return n.kids[i]; };
+            auto KN = [&](std::size_t i) -> const flat::FNode& { return p.nodes[static_cast<std::size_t>(n.kids[i])]; };
+            o << indent;
+            switch(n.op) {
+                case FOp::Const: {
+                    int c = cidx(id);
+                    o << "const IV " << v(id) << "{C[" << 2 * c << "], C[" << 2 * c + 1 << "], " << (n.c_undef ? "true" : "false") << "};";
+                    break;
+                }
+                case FOp::Arg: o << "const IV " << v(id) << "{lb[" << n.arg << "], ub[" << n.arg << "], false};"; break;
+                case FOp::Neg: o << "const IV " << v(id) << " = iv_neg(" << v(K(0)) << ");"; break;
+                case FOp::Add: o << "const IV " << v(id) << " = iv_add(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Sub: o << "const IV " << v(id) << " = iv_sub(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Mul:
+                    o << "const IV " << v(id) << " = iv_mul<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
+                      << tri_nn(KN(1)) << ", " << tri_np(KN(1)) << ", "
+                      << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
+                    break;
+                case FOp::Div:
+                    o << "const IV " << v(id) << " = iv_div<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
+                      << tri_pos(KN(1)) << ", " << tri_neg(KN(1)) << ", "
+                      << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
+                    break;
+                case FOp::Sqrt: o << "const IV " << v(id) << " = iv_sqrt<" << tri_nn(KN(0)) << ">(" << v(K(0)) << ");"; break;
+                case FOp::Square: o << "const IV " << v(id) << " = iv_square(" << v(K(0)) << ");"; break;
+                case FOp::Max: case FOp::Min: {
+                    const char* f = n.op == FOp::Max ? "iv_max" : "iv_min";
+                    std::string acc = v(K(n.kids.size() - 1));
+                    for(std::size_t i = n.kids.size() - 1; i-- > 0;) acc = std::string(f) + "(" + v(K(i)) + ", " + acc + ")";
+                    o << "const IV " << v(id) << " = " << acc << ";";
+                    break;
+                }
+                case FOp::Ite: o << "const IV " << v(id) << " = iv_ite(" << b(K(0)) << ", " << v(K(1)) << ", " << v(K(2)) << ");"; break;
+                case FOp::Acos: throw std::runtime_error("acos is not supported by the GPU code generator yet");
+                case FOp::BoolConst: o << "const IB " << b(id) << "{" << (n.b_def ? "true" : "false") << ", " << (n.b_poss ? "true" : "false") << "};"; break;
+                case FOp::Lt: o << "const IB " << b(id) << " = ib_lt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Le: o << "const IB " << b(id) << " = ib_le(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Gt: o << "const IB " << b(id) << " = ib_gt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Ge: o << "const IB " << b(id) << " = ib_ge(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Eq: o << "const IB " << b(id) << " = ib_eq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Neq: o << "const IB " << b(id) << " = ib_neq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+                case FOp::Not: o << "const IB " << b(id) << " = ib_not(" << b(K(0)) << ");"; break;
+                case FOp::Or: case FOp::And: {
+                    const char* f = n.op == FOp::Or ? "ib_or" : "ib_and";
+                    std::string acc = b(K(0));
+                    for(std::size_t i = 1; i < n.kids.size(); ++i) acc = std::string(f) + "(" + acc + ", " + b(K(i)) + ")";
+                    o << "const IB " << b(id) << " = " << acc << ";";
+                    break;
+                }
+            }
+            o << "\n";
+        }
+        return o.str();
+    }
+};
+
+} // namespace
+
+bool supported(const flat::Program& p) {
+    for(const auto& n : p.nodes) if(n.op == FOp::Acos) return false;
+    return true;
+}
+
+std::string generate(const flat::Program& p, const std::string& key) {
+    Emitter e{p, {}, {}};
+    const std::string ns = "sqd_gen_" + key;
+    std::vector<int> dyn, sta;
+    for(std::size_t i = 0; i < p.splits.size(); ++i) (p.splits[i].dynamic ? dyn : sta).push_back(static_cast<int>(i));
+    std::ostringstream body;
+
+    // constraint predicates, one function per constraint root
+    std::map<int, bool> roots;
+    for(const auto& r : p.rows) for(int c : r) roots[c] = true;
+    for(const auto& kv : roots) {
+        body << "    __device__ static IB c" << kv.first << "(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
+        body << "        (void)lb; (void)ub; (void)C;\n";
+        body << e.eval(kv.first, "        ");
+        body << "        return " << e.b(kv.first) << ";\n    }\n";
+    }
+    // bound application
+    body << "    __device__ static int apply(int k, double (&lb)[N], double (&ub)[N], const double* C) {\n";
+    body << "        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";
+    for(std::size_t k = 0; k < p.bounds.size(); ++k) {
+        const flat::FBound& bd = p.bounds[k];
+        const int t = bd.target;
+        body << "        case " << k << ": {\n" << e.eval(bd.expr, "            ");
+        const std::string f = e.v(bd.expr);
+        if(bd.honor_undef) body << "            if(" << f << ".und) return 0;\n";
+        if(bd.dir == flat::Dir::LEQ) {
+            body << "            if(" << f << ".ub < ub[" << t << "]) { ub[" << t << "] = " << f << ".ub; return " << f
+                 << ".ub < lb[" << t << "] ? -1 : " << 1 + t * 3 + 1 << "; }\n            return 0;\n";
+        } else if(bd.dir == flat::Dir::GEQ) {
+            body << "            if(" << f << ".lb > lb[" << t << "]) { lb[" << t << "] = " << f << ".lb; return " << f
+                 << ".lb > ub[" << t << "] ? -1 : " << 1 + t * 3 + 0 << "; }\n            return 0;\n";
+        } else {
+            body << "            int ch = 0;\n"
+                 << "            if(" << f << ".ub < ub[" << t << "]) { ub[" << t << "] = " << f << ".ub; ch |= 2; }\n"
+                 << "            if(" << f << ".lb > lb[" << t << "]) { lb[" << t << "] = " << f << ".lb; ch |= 1; }\n"
+                 << "            if(ub[" << t << "] < lb[" << t << "]) return -1;\n"
+                 << "            return ch ? " << 1 + t * 3 << " + (ch - 1) : 0;\n";
+        }
+        body << "        }\n";
+    }
+    body << "        default: return 0;\n        }\n    }\n";
+    // dependency lists
+    body << "    template<class Fn> __device__ static void deps(int list, Fn&& f) {\n        switch(list) {\n";
+    for(int a = 0; a < p.nargs; ++a) {
+        for(int w = 0; w < 3; ++w) {
+            const auto& l = p.update[static_cast<std::size_t>(a)][static_cast<std::size_t>(w)];
+            if(l.empty()) continue;
+            body << "        case " << a * 3 + w << ":";
+            for(int x : l) body << " f(" << x << ");";
+            body << " break;\n";
+        }
+    }
+    body << "        default: break;\n        }\n    }\n";
+    // constraint rows per static level
+    body << "    template<int L> __device__ static bool rows_violated(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
+    body << "        (void)lb; (void)ub; (void)C;\n";
+    for(std::size_t L = 0; L < sta.size(); ++L) {
+        int from = p.splits[static_cast<std::size_t>(sta[L])].arg;
+        int to = L + 1 < sta.size() ? p.splits[static_cast<std::size_t>(sta[L + 1])].arg : p.nargs;
+        body << "        if constexpr(L == " << L << ") {\n";
+        for(int r = from; r < to; ++r)
+            for(int c : p.rows[static_cast<std::size_t>(r)])
+                body << "            if(!c" << c << "(lb, ub, C).poss) return true;\n";
+        body << "        }\n";
+    }
+    body << "        return false;\n    }\n";
+    body << "    __device__ static bool all_satisfied(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
+    for(int c : p.rows[static_cast<std::size_t>(p.nargs - 1)]) body << "        if(!c" << c << "(lb, ub, C).poss) return false;\n";
+    body << "        (void)lb; (void)ub; (void)C;\n        return true;\n    }\n";
+
+    std::ostringstream o;
+    o << "// Generated from the lowered program '" << key << "' (" << p.name << "); do not edit.\n";
+    o << "#include \"kernel_set.hpp\"\n\n";
+    o << "namespace " << ns << " {\nusing namespace sqd::dev;\n\n";
+    o << "struct P {\n";
+    o << "    static constexpr int N = " << p.nargs << ";\n";
+    o << "    static constexpr int NB = " << p.bounds.size() << ";\n";
+    o << "    static constexpr int NC = " << e.consts.size() / 2 << ";\n";
+    o << "    static constexpr int NDYN = " << dyn.size() << ";\n";
+    o << "    static constexpr int NSTAT = " << sta.size() << ";\n";
+    auto table = [&](const char* name, const std::vector<int>& idx, auto get) {
+        o << "    static constexpr int " << name << "(int j) { switch(j) {";
+        for(std::size_t j = 0; j < idx.size(); ++j) o << " case " << j << ": return " << get(p.splits[static_cast<std::size_t>(idx[j])]) << ";";
+        o << " default: return 1; } }\n";
+    };
+    table("dyn_arg", dyn, [](const flat::FSplit& s) { return s.arg; });
+    table("dyn_initial", dyn, [](const flat::FSplit& s) { return static_cast<int>(s.initial); });
+    table("dyn_subs", dyn, [](const flat::FSplit& s) { return static_cast<int>(s.subs); });
+    table("st_arg", sta, [](const flat::FSplit& s) { return s.arg; });
+    table("st_subs", sta, [](const flat::FSplit& s) { return static_cast<int>(s.subs); });
+    o << body.str();
+    o << "};\n\n";
+    o << "static const double k_consts[] = {";
+    for(std::size_t i = 0; i < e.consts.size(); ++i) o << (i ? ", " : "") << dlit(e.consts[i]);
+    if(e.consts.empty()) o << "0.0, 0.0";
+    o << "};\n\n";
+    char fp[32];
+    std::snprintf(fp, sizeof fp, "0x%016llxull", static_cast<unsigned long long>(p.fingerprint()));
+    o << "static const sqd::KernelSet k_set = sqd::make_kernel_set<P>(" << fp << ", \"" << key << "\", k_consts);\n";
+    o << "static const int k_registered = (sqd::register_kernel_set(&k_set), 0);\n";
+    o << "} // namespace " << ns << "\n";
+    return o.str();
+}
+
+} // namespace codegen
+} // namespace ivarp

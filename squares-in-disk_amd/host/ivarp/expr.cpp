@@ -8,6 +8,7 @@
 #include "bounds_rules.hpp"
 #include <unordered_map>
 #include <algorithm>
+#include <limits>
 
 namespace ivarp {
 namespace fpb = fixed_point_bounds;
@@ -101,6 +102,17 @@ NodeP literal_node(const Literal& l) {
     n->lb = l.lb;
     n->ub = l.ub;
     return n;
+}
+
+Expr::Expr(int v) {
+    auto n = std::make_shared<Node>();
+    n->op = Op::Const;
+    n->exact_value = std::make_shared<exact::RInterval>(exact::RInterval::point(exact::Rat(v)));
+    n->c_lb = n->c_ub = static_cast<double>(v);
+    n->bounded = true;
+    n->lb = fpb::int_to_fp(std::numeric_limits<int>::min());
+    n->ub = fpb::int_to_fp(std::numeric_limits<int>::max());
+    node = n;
 }
 
 Pred::Pred(bool b) {

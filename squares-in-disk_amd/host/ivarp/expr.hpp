@@ -79,6 +79,9 @@ public:
     Expr() = default;
     explicit Expr(NodeP n) : node(std::move(n)) {}
     Expr(const Literal& l) : node(literal_node(l)) {} // NOLINT: implicit promotion like MathConstant
+    // Built-in integers become constants whose compile-time bounds are the range
+    // of `int` (NumberToConstant / BoundsFromIntType, math_fn/constant/number_to_constant.hpp:28-34).
+    Expr(int v); // NOLINT
     // Symbolic call: substitute x0, x1, ... by the given expressions.
     template<typename... A> Expr operator()(const A&... a) const {
         std::vector<NodeP> r{Expr(a).node...};

@@ -7,6 +7,7 @@
 #include <iostream>
 #include <iomanip>
 #include "lemmas.hpp"
+#include "ivarp/codegen.hpp"
 
 using namespace sqd_lemmas;
 
@@ -49,7 +50,19 @@ int main(int argc, char** argv) {
             }
             return 0;
         }
-        std::cerr << "usage: sqd_lower list | summary <key> | dump <key> [file] | dump-all <dir>\n";
+        if(cmd == "gen" && argc > 2) {
+            // lowered program text and generated HIP source for every lemma the GPU supports
+            for(const auto& d : all_lemmas()) {
+                auto p = lower_lemma(d);
+                { std::ofstream f(std::string(argv[2]) + "/" + d.key + ".prog"); p.write(f); }
+                if(!ivarp::codegen::supported(p)) { std::cout << "skip " << d.key << "\n"; continue; }
+                std::ofstream f(std::string(argv[2]) + "/" + d.key + ".hip");
+                f << ivarp::codegen::generate(p, d.key);
+                std::cout << "gen " << d.key << "\n";
+            }
+            return 0;
+        }
+        std::cerr << "usage: sqd_lower list | summary <key> | dump <key> [file] | dump-all <dir> | gen <dir>\n";
         return 2;
     } catch(const std::exception& e) {
         std::cerr << "sqd_lower: " << e.what() << "\n";

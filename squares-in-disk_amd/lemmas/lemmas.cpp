@@ -210,6 +210,21 @@ ConstraintSystem more_subcontainers_2() {
         F_MSC_2p1 <= 1.6_X);
 }
 
+// Small known-answer systems of the reference's prover tests.
+// ivarp/test/expected_criticals.cpp:54-61 (run with generation_count = 2)
+ConstraintSystem expected_criticals_2d() {
+    const Expr x = x0, y = x1;
+    return constraint_system(variable(x, "x", 0_Z, 2_Z), variable(y, "y", 0_Z, 2_Z),
+                             x >= y, x <= 3_Z - y, y <= 3_Z - x);
+}
+// ivarp/test/expected_criticals2.cpp:30-40 (run with generation_count 0, 1, 2)
+ConstraintSystem expected_criticals_4d() {
+    const Expr w = x0, x = x1, y = x2, z = x3;
+    return constraint_system(variable(w, "w", 0_Z, 1_Z), variable(x, "x", 0_Z, 1_Z),
+                             variable(y, "y", 0_Z, 1_Z), variable(z, "z", 0_Z, 1_Z),
+                             w + x + y + z >= 0);
+}
+
 constexpr std::uint64_t dyn(std::uint32_t i, std::uint32_t s) { return ivarp::dynamic_subdivision(i, s); }
 
 } // namespace
@@ -234,6 +249,9 @@ const std::vector<LemmaDef>& all_lemmas() {
          {dyn(128, 4), dyn(64, 4), dyn(64, 4), 32, 32}, 1056154896, 1022752013},
         {"more2", ">= 5 Subcontainers, s_n <= sigma, y_3 > 0", more_subcontainers_2,
          {dyn(128, 4), dyn(64, 4), dyn(64, 4), 32, 32, 16}, 123127968, 115680431},
+        // known-answer systems of the reference's tests; totals are for generation_count = 2
+        {"ec1", "expected criticals, 2 variables", expected_criticals_2d, {dyn(8, 2), 4}, 408, 352},
+        {"ec2", "expected criticals, 4 variables", expected_criticals_4d, {dyn(4, 4), dyn(4, 4), 8, 8}, 614868, 541696},
     };
     return defs;
 }

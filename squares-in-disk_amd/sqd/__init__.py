@@ -1,0 +1,257 @@
+"""Python binding of the MI355X prover core (libsqd.so, C ABI in include/sqd.h).
+
+This is host plumbing over the C ABI; every computation runs in the HIP
+kernels of libsqd.so.  Loading fails loudly when the library has not been built.
+
+    prog = Program.from_lemma("one")
+    with Core(prog) as core:
+        result = core.prove()          # counts, criticals, seconds
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libsqd.so"
+
+_lib = None
+
+
+class SqdError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("sqd error %d: %s" % (code, message))
+        self.code = code
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [("num_cuboids", ctypes.c_uint64), ("num_leaf_cuboids", ctypes.c_uint64),
+                ("num_critical_cuboids", ctypes.c_uint64), ("num_repeated_nodes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("generation_count", ctypes.c_uint32), ("dequeue_buffer_size", ctypes.c_uint32),
+                ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
+                ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_SZ = ctypes.c_size_t
+
+
+def lib():
+    """Load libsqd.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SqdError(-2, "libsqd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(str(LIB_PATH))
+        sig = {
+            "sqd_settings_default": (None, [ctypes.POINTER(Settings)]),
+            "sqd_program_from_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+            "sqd_program_from_lemma": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+            "sqd_program_destroy": (None, [_P]),
+            "sqd_program_text": (ctypes.c_int, [_P, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
+            "sqd_program_nargs": (ctypes.c_int, [_P]),
+            "sqd_program_has_kernels": (ctypes.c_int, [_P]),
+            "sqd_lemma_count": (ctypes.c_int, []),
+            "sqd_lemma_key": (ctypes.c_char_p, [ctypes.c_int]),
+            "sqd_lemma_title": (ctypes.c_char_p, [ctypes.c_int]),
+            "sqd_core_create": (ctypes.c_int, [ctypes.c_int, _P, ctypes.POINTER(Settings), ctypes.POINTER(_P)]),
+            "sqd_core_destroy": (None, [_P]),
+            "sqd_initial_box": (ctypes.c_int, [_P, _D, _D]),
+            "sqd_initial_queue": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
+            "sqd_nonfinal": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(Counts)]),
+            "sqd_final": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
+            "sqd_split": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
+            "sqd_prove": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _D, _D, _SZ, ctypes.POINTER(_SZ),
+                                         ctypes.POINTER(ctypes.c_double)]),
+            "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
+            "sqd_last_error": (ctypes.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the C ABI functions declared in include/sqd.h."""
+    return ["sqd_settings_default", "sqd_program_from_text", "sqd_program_from_lemma", "sqd_program_destroy",
+            "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
+            "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
+            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_rounded_op",
+            "sqd_last_error"]
+
+
+def _check(rc):
+    if rc != 0:
+        raise SqdError(rc, lib().sqd_last_error().decode())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def lemmas():
+    L = lib()
+    return [(L.sqd_lemma_key(i).decode(), L.sqd_lemma_title(i).decode()) for i in range(L.sqd_lemma_count())]
+
+
+def default_settings(**kw):
+    s = Settings()
+    lib().sqd_settings_default(ctypes.byref(s))
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+class Program:
+    """A lowered program bound to its compiled kernels."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_lemma(cls, key):
+        h = _P()
+        _check(lib().sqd_program_from_lemma(key.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_text(cls, text):
+        h = _P()
+        _check(lib().sqd_program_from_text(text.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @property
+    def nargs(self):
+        return lib().sqd_program_nargs(self._h)
+
+    @property
+    def has_kernels(self):
+        return bool(lib().sqd_program_has_kernels(self._h))
+
+    def text(self):
+        n = _SZ()
+        _check(lib().sqd_program_text(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().sqd_program_text(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def close(self):
+        if self._h:
+            lib().sqd_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Core:
+    """One prover core on one device (sqd_core)."""
+
+    def __init__(self, program, device=0, settings=None):
+        self.program = program
+        self.N = program.nargs
+        self._h = _P()
+        s = settings if settings is not None else default_settings()
+        _check(lib().sqd_core_create(device, program._h, ctypes.byref(s), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().sqd_core_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def initial_box(self):
+        lb = np.zeros(self.N)
+        ub = np.zeros(self.N)
+        _check(lib().sqd_initial_box(self._h, _dptr(lb), _dptr(ub)))
+        return lb, ub
+
+    def initial_queue(self, capacity=1 << 20):
+        lb = np.zeros((self.N, capacity))
+        ub = np.zeros((self.N, capacity))
+        n = _SZ()
+        c = Counts()
+        _check(lib().sqd_initial_queue(self._h, _dptr(lb), _dptr(ub), capacity, ctypes.byref(n), ctypes.byref(c)))
+        m = min(n.value, capacity)
+        return lb[:, :m].copy(), ub[:, :m].copy(), c.as_dict()
+
+    def nonfinal(self, lb, ub):
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        hl = np.zeros_like(lb)
+        hu = np.zeros_like(ub)
+        has = np.zeros(n, dtype=np.uint8)
+        c = Counts()
+        _check(lib().sqd_nonfinal(self._h, _dptr(lb), _dptr(ub), n, _dptr(hl), _dptr(hu),
+                                   has.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(c)))
+        return hl, hu, has, c.as_dict()
+
+    def final(self, lb, ub, capacity=1 << 20):
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        cl = np.zeros((self.N, capacity))
+        cu = np.zeros((self.N, capacity))
+        k = _SZ()
+        c = Counts()
+        _check(lib().sqd_final(self._h, _dptr(lb), _dptr(ub), n, _dptr(cl), _dptr(cu), capacity, ctypes.byref(k),
+                                ctypes.byref(c)))
+        m = min(k.value, capacity)
+        return cl[:, :m].copy(), cu[:, :m].copy(), c.as_dict()
+
+    def split(self, lb, ub, capacity=1 << 20):
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        ol = np.zeros((self.N, capacity))
+        ou = np.zeros((self.N, capacity))
+        k = _SZ()
+        c = Counts()
+        _check(lib().sqd_split(self._h, _dptr(lb), _dptr(ub), n, _dptr(ol), _dptr(ou), capacity, ctypes.byref(k),
+                                ctypes.byref(c)))
+        m = min(k.value, capacity)
+        return ol[:, :m].copy(), ou[:, :m].copy(), c.as_dict()
+
+    def prove(self, critical_capacity=1 << 16):
+        c = Counts()
+        cl = np.zeros((self.N, critical_capacity))
+        cu = np.zeros((self.N, critical_capacity))
+        k = _SZ()
+        secs = ctypes.c_double()
+        _check(lib().sqd_prove(self._h, ctypes.byref(c), _dptr(cl), _dptr(cu), critical_capacity, ctypes.byref(k),
+                                ctypes.byref(secs)))
+        m = min(k.value, critical_capacity)
+        return {"counts": c.as_dict(), "crit_lb": cl[:, :m].copy(), "crit_ub": cu[:, :m].copy(),
+                "n_critical": int(k.value), "seconds": secs.value}
+
+
+def rounded_op(op, a, b, device=0):
+    """Device RD/RU of a op b (op: 'add', 'sub', 'mul', 'div', 'sqrt')."""
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}[op]
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    lo = np.zeros_like(a)
+    hi = np.zeros_like(a)
+    _check(lib().sqd_rounded_op(device, code, a.size, _dptr(a), _dptr(b), _dptr(lo), _dptr(hi)))
+    return lo, hi
