@@ -93,10 +93,9 @@ __device__ __forceinline__ double approx_sqrt(double a) {
     return __builtin_sqrt(a);
 }
 
-// RD(a / b)
-__device__ inline double rd_div(double a, double b) {
+// RD(a / b) for normal operands whose quotient is normal (|a|, |b| in [0.5, 1)).
+__device__ inline double rd_div_core(double a, double b) {
     double q = approx_div(a, b);
-    if(!is_finite(a) || !is_finite(b) || a == 0.0 || b == 0.0 || !is_finite(q) || q == 0.0) return q;
     const bool bneg = b < 0.0;
     // q too large while (a - q b)/b < 0
     #pragma unroll 1
@@ -112,13 +111,24 @@ __device__ inline double rd_div(double a, double b) {
     }
     return q;
 }
+// RD(a / b).  The operands are normalised to mantissas in [0.5, 1) so that the
+// residual of the correction is exact even for subnormal inputs; the exponent is
+// restored with ldexp, which rounds in the current (downward) mode when the
+// result is subnormal or overflows.  The subnormal grid is a subset of the grid
+// of the scaled quotient, so this second downward rounding equals one rounding.
+__device__ inline double rd_div(double a, double b) {
+    if(!is_finite(a) || !is_finite(b) || a == 0.0 || b == 0.0) return approx_div(a, b);  // exact IEEE specials
+    int ea, eb;
+    const double ma = __builtin_frexp(a, &ea);
+    const double mb = __builtin_frexp(b, &eb);
+    return __builtin_ldexp(rd_div_core(ma, mb), ea - eb);
+}
 // RU(a / b) = -RD(-a / b)
 __device__ inline double ru_div(double a, double b) { return -rd_div(-a, b); }
 
-// RD(sqrt(a))
-__device__ inline double rd_sqrt(double a) {
+// RD(sqrt(a)) for a in [0.5, 2).
+__device__ inline double rd_sqrt_core(double a) {
     double s = approx_sqrt(a);
-    if(!(a > 0.0) || !is_finite(a) || !is_finite(s)) return s;
     #pragma unroll 1
     for(int i = 0; i < 4; ++i) {
         if(fms_rd(a, s, s) < 0.0) s = next_down(s); else break;
@@ -130,11 +140,23 @@ __device__ inline double rd_sqrt(double a) {
     }
     return s;
 }
+// RD(sqrt(a)): a = m 2^(2k) with m in [0.5, 2), sqrt(a) = sqrt(m) 2^k exactly.
+__device__ inline double rd_sqrt(double a) {
+    if(!(a > 0.0) || !is_finite(a)) return approx_sqrt(a);  // 0, -0, +inf, negative, NaN
+    int e;
+    double m = __builtin_frexp(a, &e);
+    if(e & 1) { m = __builtin_ldexp(m, 1); e -= 1; }
+    return __builtin_ldexp(rd_sqrt_core(m), e / 2);
+}
 // RU(sqrt(a))
 __device__ inline double ru_sqrt(double a) {
-    double s = rd_sqrt(a);
-    if(!(a > 0.0) || !is_finite(a) || !is_finite(s)) return s;
-    return fms_rd(a, s, s) > 0.0 ? next_up(s) : s;
+    if(!(a > 0.0) || !is_finite(a)) return approx_sqrt(a);
+    int e;
+    double m = __builtin_frexp(a, &e);
+    if(e & 1) { m = __builtin_ldexp(m, 1); e -= 1; }
+    double s = rd_sqrt_core(m);
+    if(fms_rd(m, s, s) > 0.0) s = next_up(s);
+    return __builtin_ldexp(s, e / 2);
 }
 
 } // namespace dev

@@ -19,7 +19,8 @@ def _bits(a):
 
 def _edge_values():
     v = [0.0, -0.0, 1.0, -1.0, 0.5, 2.0, 3.0, 5.0, 1e-300, -1e-300, 1e300, -1e300, 5e-324, -5e-324,
-         2.2250738585072014e-308, np.inf, -np.inf, 1.2649110640673517, 0.295, 1.6, 0.1, 1 / 3]
+         2.2250738585072014e-308, 3e-310, -7.5e-321, 1e10, -3e15, 1.7976931348623157e308, np.inf, -np.inf,
+         1.2649110640673517, 0.295, 1.6, 0.1, 1 / 3]
     return np.array(v)
 
 
