@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the GPU prover core on one MI355X.
+
+Workload: the One Subcontainer Lemma (src/one_subcontainer.cpp in the
+reference) proven in plain binary64 interval arithmetic with the reference
+split policy: 128 initial dynamic cuboids on s_1, a 256 x 256 static grid on
+(h_1, s_n), 8 generations -- 2,869,888 cuboids and 2,858,678 leaves per proof.
+
+A step is one whole proof (initial box, initial queue, static levels, hull
+reduction, final pass, dynamic splits) on the device; the program and its
+constants are resident in HBM before timing starts.  Every step's counts are
+checked against the published ones.  The line printed carries
+
+  value       cuboids/s over the timed steps (wall clock around the steps)
+  latency_ms  p50/p99 of the per-proof wall time
+  roofline    the dominant kernel (largest summed device time), its
+              algorithmic binary64 operations per launch (op-counted
+              restatement of the same algorithm, per kernel kind) over its
+              mean launch time from HIP events on the core's stream, against
+              the MI355X FP64 vector peak
+  cpu_baseline the CPU restatement (oracle) timed on this host's cores on the
+              same proof
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "squares-in-disk_amd"))
+
+METRIC = "cuboids/sec + p50/p99 proof-job latency, FP32-first batched vs plain FP64, 1 GPU"
+FP64_PEAK_TFLOPS = 78.6          # MI355X binary64 vector peak
+PUBLISHED = {                    # (cuboids, leaves) from the reference's prover output
+    "top1": (128, 128), "top2": (168, 168), "one": (2869888, 2858678), "two": (7181696, 7069587),
+    "three": (3549856, 3440786), "four": (37245328, 35026303), "five": (3392, 3188), "six": (14848, 13928),
+    "seven": (1808, 1703),
+}
+WORKLOADS = {
+    "one": "one_subcontainer_plain_fp64",
+    "two": "two_subcontainer_plain_fp64",
+    "three": "three_subcontainer_plain_fp64",
+    "four": "four_subcontainer_plain_fp64",
+}
+MODES = {0: "NONFINAL", 1: "FINAL", 2: "SPLIT", 3: "INIT"}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--lemma", default="one", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed oracle runs for the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=None,
+                    help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
+    return ap.parse_args()
+
+
+def percentile(xs, q):
+    s = sorted(xs)
+    if not s:
+        return None
+    k = (len(s) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(s) - 1)
+    return s[lo] + (s[hi] - s[lo]) * (k - lo)
+
+
+def cpu_baseline(program_text, runs):
+    """Op counts per kernel kind and the CPU restatement's throughput on this host."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.NamedTemporaryFile("w", suffix=".prog", delete=False) as f:
+        f.write(program_text)
+        path = f.name
+    try:
+        counted = pyoracle.run_cli(path, threads=threads, flops=True, timeout=600)
+        timed = [pyoracle.run_cli(path, threads=threads, timeout=600) for _ in range(runs)]
+    finally:
+        os.unlink(path)
+    secs = statistics.median(r["seconds"] for r in timed)
+    return counted, {
+        "value": timed[0]["num_cuboids"] / secs,
+        "unit": "cuboids/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "the whole proof (%d cuboids), median of %d runs, %d threads, %.3f s each"
+                  % (timed[0]["num_cuboids"], runs, threads, secs),
+    }
+
+
+def read_traffic(path, kernel_name):
+    """Bytes per launch of `kernel_name` from a traffic summary (key = value lines)."""
+    if not path or not Path(path).exists():
+        return None
+    vals = {}
+    for line in Path(path).read_text().splitlines():
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("kernel") != kernel_name or "bytes_per_launch" not in vals:
+        return None
+    return float(vals["bytes_per_launch"])
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py: this prover core runs on one GPU; multi-GPU runs (--gpus %d, WORLD_SIZE %d) are not "
+              "part of this build" % (a.gpus, world), file=sys.stderr)
+        sys.exit(2)
+    import sqd
+
+    prog = sqd.Program.from_lemma(a.lemma)
+    expected = PUBLISHED[a.lemma]
+    core = sqd.Core(prog, settings=sqd.default_settings(generation_count=8))
+    try:
+        for _ in range(a.warmup):
+            r = core.prove()
+            c = r["counts"]
+            if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+                raise SystemExit("count mismatch in warmup: %s" % c)
+        core.set_profiling(True)
+        lat = []
+        t0 = time.perf_counter()
+        total_cuboids = 0
+        for _ in range(a.steps):
+            s0 = time.perf_counter()
+            r = core.prove()
+            lat.append(time.perf_counter() - s0)
+            c = r["counts"]
+            if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+                raise SystemExit("count mismatch: %s" % c)
+            total_cuboids += c["num_cuboids"]
+        elapsed = time.perf_counter() - t0
+        stats = core.kernel_stats()
+        core.set_profiling(False)
+    finally:
+        core.close()
+
+    counted, cpu = (None, None)
+    if not a.no_cpu_baseline:
+        counted, cpu = cpu_baseline(prog.text(), a.cpu_runs)
+
+    dom = max((s for s in stats if s["mode"] >= 0), key=lambda s: s["milliseconds"])
+    mean_ms = dom["milliseconds"] / dom["launches"]
+    roofline = None
+    if counted is not None:
+        ph = counted["phases"].get("%d.%d" % (dom["mode"], dom["level"]), [0, 0])
+        flops_per_proof, cuboids_per_proof = ph
+        launches_per_proof = dom["launches"] / a.steps
+        flops_per_launch = flops_per_proof / launches_per_proof
+        achieved = flops_per_launch / (mean_ms * 1e-3) / 1e12
+        traffic = read_traffic(a.traffic, dom["name"])
+        roofline = {
+            "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "kernel": dom["name"], "mean_launch_ms": mean_ms, "launches_per_proof": launches_per_proof,
+            "flops_per_launch": flops_per_launch,
+            "cuboids_per_launch": cuboids_per_proof / launches_per_proof,
+            "flops_per_cuboid": flops_per_proof / cuboids_per_proof if cuboids_per_proof else None,
+        }
+    device_ms = sum(s["milliseconds"] for s in stats) / a.steps
+    line = {
+        "metric": METRIC,
+        "value": total_cuboids / elapsed,
+        "unit": "cuboids/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "lemma-defined (no input data)",
+        "config": {"workload": WORKLOADS[a.lemma], "lemma": a.lemma, "arithmetic": "plain binary64 intervals",
+                   "split_policy": "reference", "generations": 8, "cuboids_per_proof": expected[0],
+                   "leaves_per_proof": expected[1]},
+        "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
+        "device_ms_per_step": device_ms,
+        "kernels": [{k: s[k] for k in ("name", "launches", "milliseconds")} for s in
+                    sorted(stats, key=lambda s: -s["milliseconds"])],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

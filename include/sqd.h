@@ -55,7 +55,7 @@ typedef struct sqd_counts {
 typedef struct sqd_settings {
     uint32_t generation_count;     /* generations of dynamic splits (reference default 8) */
     uint32_t dequeue_buffer_size;  /* dynamic cuboids per round; default chosen per program */
-    uint64_t frontier_capacity;    /* cuboids the device frontier can hold; default 1<<24 */
+    uint64_t frontier_capacity;    /* cuboids the device frontier can hold; default 1<<23 */
     uint64_t level_bytes;          /* bytes per level buffer; default 1<<30 */
     uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
     uint32_t grid_blocks;          /* blocks per level launch; default 8 per CU */
@@ -120,6 +120,25 @@ int sqd_split(sqd_core* core, const double* lb, const double* ub, size_t n,
  * spent on the device loop. */
 int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_ub,
               size_t capacity, size_t* n_critical, double* seconds);
+
+/* ---------------------------------------------------------------- timing */
+
+/* Device time per kernel kind, from HIP events recorded on the core's stream
+ * around every launch while profiling is on.  No counterpart in the reference
+ * (its ProofInformation, ivarp/include/ivarp/prover/proof_information.hpp,
+ * holds counts only); used by the throughput measurement. */
+typedef struct sqd_kernel_stat {
+    char name[48];        /* "static_level<NONFINAL,1>", "hull_finalize", ... */
+    int32_t mode;         /* 0 non-final, 1 final, 2 split, 3 initial queue, -1 other kernels */
+    int32_t level;        /* static or dynamic level; -1 for other kernels */
+    uint64_t launches;
+    double milliseconds;  /* summed event time over the launches */
+} sqd_kernel_stat;
+
+/* enable != 0 turns profiling on and clears the statistics; 0 turns it off. */
+int sqd_core_set_profiling(sqd_core* core, int enable);
+/* Copies up to `capacity` entries (ordered by kind); *count receives the number of kinds. */
+int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t capacity, size_t* count);
 
 /* ---------------------------------------------------------------- tests */
 

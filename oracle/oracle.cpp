@@ -21,12 +21,39 @@
 namespace {
 
 // ---------------------------------------------------------------- arithmetic
+// Operation counting for the roofline figures: every binary64 add, sub, mul,
+// div and sqrt is counted once, attributed to the phase that executes it.  A
+// phase is (mode, level) with mode 0 non-final static level, 1 final static
+// level, 2 dynamic split level, 3 initial-queue level (the kernel kinds of the
+// device path); kPhaseOther collects everything else (initial box).
+constexpr int kPhaseLevels = 16;
+constexpr int kPhaseOther = 4 * kPhaseLevels;
+constexpr int kPhases = kPhaseOther + 1;
 static std::atomic<std::uint64_t> g_flops{0};
+static std::atomic<std::uint64_t> g_phase_flops[kPhases];
+static std::atomic<std::uint64_t> g_phase_units[kPhases];
 static bool g_count_flops = false;
-thread_local std::uint64_t t_flops = 0;
+thread_local std::uint64_t t_phase_flops[kPhases];
+thread_local std::uint64_t t_phase_units[kPhases];
+thread_local int t_phase = kPhaseOther;
 
 inline double opq(double x) { __asm__ volatile("" : "+x"(x)); return x; }
-inline void count(int k) { if(g_count_flops) t_flops += static_cast<std::uint64_t>(k); }
+inline void count(int k) { if(g_count_flops) t_phase_flops[t_phase] += static_cast<std::uint64_t>(k); }
+inline void enter_phase(int mode, std::size_t level, std::uint64_t units) {
+    t_phase = level < static_cast<std::size_t>(kPhaseLevels) ? mode * kPhaseLevels + static_cast<int>(level) : kPhaseOther;
+    if(g_count_flops) t_phase_units[t_phase] += units;
+}
+// Moves this thread's counts into the global totals.
+void flush_thread_counts() {
+    for(int i = 0; i < kPhases; ++i) {
+        g_flops += t_phase_flops[i];
+        g_phase_flops[i] += t_phase_flops[i];
+        g_phase_units[i] += t_phase_units[i];
+        t_phase_flops[i] = 0;
+        t_phase_units[i] = 0;
+    }
+    t_phase = kPhaseOther;
+}
 
 inline double rd_add(double a, double b) { count(1); return opq(opq(a) + opq(b)); }
 inline double ru_add(double a, double b) { return -rd_add(-a, -b); }
@@ -501,28 +528,30 @@ struct Core {
     }
     // Static enumeration (cpu_prover_core.hpp:221-256); `leaf` is called for each surviving leaf.
     template<typename Leaf>
-    void enumerate(Evaluator& ev, const Cuboid& e, std::size_t level, Counts& cnt, Leaf&& leaf) const {
+    void enumerate(Evaluator& ev, const Cuboid& e, std::size_t level, int mode, Counts& cnt, Leaf&& leaf) const {
         if(level == static_splits.size()) { cnt.leaves++; leaf(e); return; }
         const PSplit& s = p.splits[static_cast<std::size_t>(static_splits[level])];
         const int arg = s.arg;
         const int subs = static_cast<int>(s.subs);
         const int row_end = level + 1 < static_splits.size()
                             ? p.splits[static_cast<std::size_t>(static_splits[level + 1])].arg : p.N;
+        enter_phase(mode, level, static_cast<std::uint64_t>(subs));
         Splitter sp(e.lb[static_cast<std::size_t>(arg)], e.ub[static_cast<std::size_t>(arg)], subs);
         for(int i = 0; i < subs; ++i) {
+            enter_phase(mode, level, 0);
             Cuboid q = e;
             q.lb[static_cast<std::size_t>(arg)] = sp.point(i);
             q.ub[static_cast<std::size_t>(arg)] = sp.point(i + 1);
             if(propagate(ev, q, arg, static_cast<std::size_t>(p.N), 4)) { cnt.leaves++; continue; }
             if(violates_rows(ev, q, arg, row_end)) { cnt.leaves++; continue; }
-            enumerate(ev, q, level + 1, cnt, leaf);
+            enumerate(ev, q, level + 1, mode, cnt, leaf);
         }
         cnt.cuboids += static_cast<std::uint64_t>(subs);
     }
     // Non-final pass with max_iterations_per_node = 1: the hull of surviving leaves.
     bool nonfinal(Evaluator& ev, const Cuboid& e, Cuboid& hull, Counts& cnt) const {
         bool empty = true;
-        enumerate(ev, e, 0, cnt, [&](const Cuboid& leaf) {
+        enumerate(ev, e, 0, 0, cnt, [&](const Cuboid& leaf) {
             if(empty) { hull = leaf; empty = false; return; }
             for(int a = 0; a < p.N; ++a) {
                 std::size_t i = static_cast<std::size_t>(a);
@@ -535,7 +564,7 @@ struct Core {
     }
     template<typename Crit>
     void final_pass(Evaluator& ev, const Cuboid& e, Counts& cnt, Crit&& crit) const {
-        enumerate(ev, e, 0, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); });
+        enumerate(ev, e, 0, 1, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); });
     }
     // Dynamic split (pd_impl_split_into.hpp:29-75) or initial split (proof_driver.hpp:320-391).
     template<typename Out>
@@ -548,8 +577,11 @@ struct Core {
         }
         const PSplit& s = p.splits[static_cast<std::size_t>(dynamic_splits[level])];
         const int n = static_cast<int>(initial ? s.initial : s.subs);
+        const int mode = initial ? 3 : 2;
+        enter_phase(mode, level, static_cast<std::uint64_t>(n));
         Splitter sp(e.lb[static_cast<std::size_t>(s.arg)], e.ub[static_cast<std::size_t>(s.arg)], n);
         for(int i = 0; i < n; ++i) {
+            enter_phase(mode, level, 0);
             cnt.cuboids++;
             Cuboid q = e;
             q.lb[static_cast<std::size_t>(s.arg)] = sp.point(i);
@@ -616,11 +648,9 @@ void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations
     std::mutex res_m;
     std::vector<std::thread> ts;
     std::vector<Counts> counts(static_cast<std::size_t>(threads));
-    std::vector<std::uint64_t> flops(static_cast<std::size_t>(threads), 0);
     for(int t = 0; t < threads; ++t) {
         ts.emplace_back([&, t] {
             RoundDownGuard g;
-            t_flops = 0;
             Evaluator ev(p);
             Counts& cnt = counts[static_cast<std::size_t>(t)];
             std::vector<Cuboid> batch, produced;
@@ -643,12 +673,11 @@ void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations
                 batch.clear();
                 f.push_done(produced);
             }
-            flops[static_cast<std::size_t>(t)] = t_flops;
+            flush_thread_counts();
         });
     }
     for(auto& t : ts) t.join();
     for(const auto& c : counts) res.counts += c;
-    for(auto x : flops) g_flops += x;
 }
 
 thread_local std::string t_error;
@@ -658,8 +687,8 @@ struct Handle {
 };
 
 template<typename F> int guarded(F&& f) {
-    try { f(); return 0; }
-    catch(const std::exception& e) { t_error = e.what(); return -1; }
+    try { f(); flush_thread_counts(); return 0; }
+    catch(const std::exception& e) { flush_thread_counts(); t_error = e.what(); return -1; }
 }
 
 void to_out(const Counts& c, OracleCounts* o) {
@@ -843,7 +872,17 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
 }
 
 std::uint64_t oracle_flop_count(void) { return g_flops.load(); }
-void oracle_flop_reset(void) { g_flops = 0; }
+void oracle_flop_reset(void) {
+    g_flops = 0;
+    for(int i = 0; i < kPhases; ++i) { g_phase_flops[i] = 0; g_phase_units[i] = 0; }
+}
+int oracle_phase_counts(int mode, int level, std::uint64_t* flops, std::uint64_t* units) {
+    if(mode < 0 || mode > 4 || level < 0 || level >= kPhaseLevels || !flops || !units) return -1;
+    const int ph = mode == 4 ? kPhaseOther : mode * kPhaseLevels + level;
+    *flops = g_phase_flops[ph].load();
+    *units = g_phase_units[ph].load();
+    return 0;
+}
 void oracle_flop_enable(int on) { g_count_flops = on != 0; }
 
 } // extern "C"

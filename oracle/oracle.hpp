@@ -74,4 +74,8 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
 std::uint64_t oracle_flop_count(void);
 void oracle_flop_reset(void);
 void oracle_flop_enable(int on);
+// The same count for one phase, and the number of child cuboids the phase
+// processed: mode 0 non-final static level, 1 final static level, 2 dynamic split
+// level, 3 initial-queue level, 4 everything else (level ignored).
+int oracle_phase_counts(int mode, int level, std::uint64_t* flops, std::uint64_t* units);
 }

@@ -1,6 +1,7 @@
 // Command-line driver of the CPU oracle (test infrastructure only).
 //   sqd_oracle <program-file> [--threads T] [--generations G] [--flops] [--slice FIRST LAST]
-// Prints one JSON line with the counts, the wall time and the thread count.
+// Prints one JSON line with the counts, the wall time, the thread count and,
+// with --flops, the binary64 operation counts (total and per phase).
 #include "oracle.hpp"
 #include <chrono>
 #include <cstdio>
@@ -39,11 +40,23 @@ int main(int argc, char** argv) {
     auto t1 = std::chrono::steady_clock::now();
     if(rc != 0) { std::fprintf(stderr, "run: %s\n", oracle_last_error()); return 1; }
     double s = std::chrono::duration<double>(t1 - t0).count();
+    // per-phase operation and child-cuboid counts: "mode.level": [flops, cuboids]
+    std::string phases;
+    for(int m = 0; m < 5 && flops; ++m)
+        for(int l = 0; l < (m == 4 ? 1 : 16); ++l) {
+            std::uint64_t pf = 0, pu = 0;
+            if(oracle_phase_counts(m, l, &pf, &pu) != 0 || (pf == 0 && pu == 0)) continue;
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "%s\"%d.%d\": [%llu, %llu]", phases.empty() ? "" : ", ", m, l,
+                          static_cast<unsigned long long>(pf), static_cast<unsigned long long>(pu));
+            phases += buf;
+        }
     std::printf("{\"num_cuboids\": %llu, \"num_leaf_cuboids\": %llu, \"num_critical_cuboids\": %llu, "
-                "\"seconds\": %.6f, \"threads\": %d, \"flops\": %llu, \"initial_queue\": %llu}\n",
+                "\"seconds\": %.6f, \"threads\": %d, \"flops\": %llu, \"initial_queue\": %llu, "
+                "\"phases\": {%s}}\n",
                 static_cast<unsigned long long>(c.num_cuboids), static_cast<unsigned long long>(c.num_leaf_cuboids),
                 static_cast<unsigned long long>(c.num_critical_cuboids), s, threads,
-                static_cast<unsigned long long>(oracle_flop_count()), static_cast<unsigned long long>(iql));
+                static_cast<unsigned long long>(oracle_flop_count()), static_cast<unsigned long long>(iql), phases.c_str());
     oracle_free(h);
     return 0;
 }

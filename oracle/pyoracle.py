@@ -61,6 +61,7 @@ def lib():
             "oracle_flop_count": (_U64, []),
             "oracle_flop_reset": (None, []),
             "oracle_flop_enable": (None, [ctypes.c_int]),
+            "oracle_phase_counts": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

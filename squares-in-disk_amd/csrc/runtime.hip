@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -147,19 +149,91 @@ struct sqd_core {
     unsigned long long k_max = 0;    // dynamic cuboids per round
     unsigned long long split_fanout = 1;
     std::vector<std::uint32_t> st_subs, dyn_subs, dyn_initial;
+    // profiling: events around each launch, resolved at the next stream sync
+    struct Pending { int kind; hipEvent_t a, b; };
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::size_t ev_used = 0;
+    std::vector<Pending> pending;
+    std::map<int, sqd_kernel_stat> stats;
 };
 
 namespace sqd {
 namespace {
 
+// Kernel kinds for the timing statistics: level kernels are mode * 16 + level.
+enum : int { KIND_HULL_FINALIZE = 100, KIND_FINAL_NOSTATIC, KIND_INIT_HULL, KIND_HULL_DECODE, KIND_INITIAL_BOX,
+             KIND_SET_COUNT };
+
+sqd_kernel_stat make_stat(int kind) {
+    sqd_kernel_stat s{};
+    static const char* modes[] = {"NONFINAL", "FINAL", "SPLIT", "INIT"};
+    if(kind < 64) {
+        s.mode = kind / 16;
+        s.level = kind % 16;
+        std::snprintf(s.name, sizeof s.name, "%s<%s,%d>", s.mode <= 1 ? "static_level" : "dynamic_level",
+                      modes[s.mode], s.level);
+    } else {
+        s.mode = -1;
+        s.level = -1;
+        const char* n = kind == KIND_HULL_FINALIZE ? "hull_finalize" : kind == KIND_FINAL_NOSTATIC ? "final_nostatic"
+                      : kind == KIND_INIT_HULL ? "init_hull" : kind == KIND_HULL_DECODE ? "hull_decode"
+                      : kind == KIND_INITIAL_BOX ? "initial_box" : "set_count";
+        std::snprintf(s.name, sizeof s.name, "%s", n);
+    }
+    return s;
+}
+
+hipEvent_t next_event(sqd_core* c) {
+    if(c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Launches through `f` (returning hipError_t); with profiling on, brackets the
+// launch with two events on the core's stream.
+template<typename F> void launch(sqd_core* c, int kind, const char* what, F&& f) {
+    hipEvent_t a = nullptr;
+    if(c->profiling) {
+        a = next_event(c);
+        hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+    }
+    hip_check(f(), what);
+    if(c->profiling) {
+        hipEvent_t b = next_event(c);
+        hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+        c->pending.push_back({kind, a, b});
+    }
+}
+
+// Waits for the core's stream and folds the finished launches into the statistics.
+void sync(sqd_core* c) {
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for(const auto& p : c->pending) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        auto it = c->stats.find(p.kind);
+        if(it == c->stats.end()) it = c->stats.emplace(p.kind, make_stat(p.kind)).first;
+        it->second.launches += 1;
+        it->second.milliseconds += ms;
+    }
+    c->pending.clear();
+    c->ev_used = 0;
+}
+
 void set_count(sqd_core* c, unsigned long long* p, unsigned long long v) {
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c->stream, p, v);
-    hip_check(hipGetLastError(), "k_set_u64");
+    launch(c, KIND_SET_COUNT, "k_set_u64", [&] {
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, c->stream, p, v);
+        return hipGetLastError();
+    });
 }
 unsigned long long get_count(sqd_core* c, unsigned long long* p) {
     unsigned long long v = 0;
     hip_check(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync count");
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    sync(c);
     return v;
 }
 void reset_counters(sqd_core* c) {
@@ -168,7 +242,7 @@ void reset_counters(sqd_core* c) {
 sqd_counts read_counters(sqd_core* c) {
     dev::DeviceCounters h{};
     hip_check(hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy counters");
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    sync(c);
     if(h.overflow) throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
     sqd_counts r;
     r.num_cuboids = h.cuboids;
@@ -214,7 +288,7 @@ void run_static(sqd_core* c, int mode, const dev::BoxList& in, unsigned long lon
         dev::BoxList src = prev;
         if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
         dev::LevelArgs a = level_args(c, src, prev_base, prev_fixed, out);
-        hip_check(c->ks->static_level(mode, L, a, c->grid, c->stream), "static level launch");
+        launch(c, mode * 16 + L, "static level launch", [&] { return c->ks->static_level(mode, L, a, c->grid, c->stream); });
         prev = c->lvl[L % 2].view();
         prev_base = 0;
         prev_fixed = 0;
@@ -237,7 +311,7 @@ void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long lo
         src.origin = nullptr;
         src.leaf = nullptr;
         dev::LevelArgs a = level_args(c, src, 0, prev_fixed, o);
-        hip_check(c->ks->dynamic_level(mode, J, a, c->grid, c->stream), "dynamic level launch");
+        launch(c, mode * 16 + J, "dynamic level launch", [&] { return c->ks->dynamic_level(mode, J, a, c->grid, c->stream); });
         prev = c->lvl[J % 2].view();
         prev_fixed = 0;
     }
@@ -257,7 +331,7 @@ void upload_soa(sqd_core* c, DevList& l, const double* lb, const double* ub, uns
         hip_check(hipMemcpyAsync(l.depth, d.data(), n * sizeof(unsigned), hipMemcpyHostToDevice, c->stream), "upload depth");
     }
     set_count(c, l.count, n);
-    hip_check(hipStreamSynchronize(c->stream), "sync");
+    sync(c);
 }
 void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* lb, double* ub, unsigned long long stride) {
     for(int a = 0; a < c->N; ++a) {
@@ -266,12 +340,14 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
         hip_check(hipMemcpyAsync(ub + static_cast<std::size_t>(a) * stride, l.ub + static_cast<std::size_t>(a) * l.cap,
                                  n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download ub");
     }
-    hip_check(hipStreamSynchronize(c->stream), "sync");
+    sync(c);
 }
 
 void compute_initial_box(sqd_core* c) {
-    hip_check(c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, c->stream), "initial box");
-    hip_check(hipStreamSynchronize(c->stream), "sync");
+    launch(c, KIND_INITIAL_BOX, "initial box", [&] {
+        return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, c->stream);
+    });
+    sync(c);
 }
 
 // Initial queue onto the (empty) stack; returns its length.
@@ -285,25 +361,35 @@ unsigned long long build_initial_queue(sqd_core* c) {
     return get_count(c, c->stack.count);
 }
 
+void init_hull(sqd_core* c, unsigned long long K) {
+    launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
+        hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, K * static_cast<unsigned long long>(c->N));
+        return hipGetLastError();
+    });
+}
+
 // One round over the top K cuboids of the stack; returns the new stack size.
 unsigned long long run_round(sqd_core* c, unsigned long long S, unsigned long long K) {
     const unsigned long long base = S - K;
     const int N = c->N;
     dev::BoxList items = c->stack.view();
     if(c->ks->NSTAT > 0) {
-        hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, K * static_cast<unsigned long long>(N));
-        hip_check(hipGetLastError(), "k_init_hull");
+        init_hull(c, K);
         run_static(c, dev::NONFINAL, items, base, K, c->crit.view());
     }
     set_count(c, c->fin.count, 0);
     set_count(c, c->spl.count, 0);
-    hip_check(c->ks->hull_finalize(items, base, K, c->d_hull, c->fin.view(), c->spl.view(), c->d_counters,
-                                   c->settings.generation_count, c->stream), "hull finalize");
+    launch(c, KIND_HULL_FINALIZE, "hull finalize", [&] {
+        return c->ks->hull_finalize(items, base, K, c->d_hull, c->fin.view(), c->spl.view(), c->d_counters,
+                                    c->settings.generation_count, c->stream);
+    });
     set_count(c, c->stack.count, base);
     // final pass over hulls at the last generation
     if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
-    else hip_check(c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
-                                         static_cast<unsigned>((K + 255) / 256), c->stream), "final");
+    else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
+        return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters, static_cast<unsigned>((K + 255) / 256),
+                                     c->stream);
+    });
     // dynamic split of the other hulls, children pushed onto the stack
     dev::BoxList spl = c->spl.view();
     run_dynamic(c, dev::SPLIT, spl, 0, c->stack.view());
@@ -452,6 +538,7 @@ void sqd_core_destroy(sqd_core* c) {
     for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_counters),
                    static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
         if(p) (void)hipFree(p);
+    for(hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -496,11 +583,12 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
                 }
             upload_soa(c, c->stack, clb.data(), cub.data(), m, 0);
             if(c->ks->NSTAT > 0) {
-                hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, m * static_cast<unsigned long long>(c->N));
-                hip_check(hipGetLastError(), "k_init_hull");
+                init_hull(c, m);
                 run_static(c, dev::NONFINAL, c->stack.view(), 0, m, c->crit.view());
             }
-            hip_check(c->ks->hull_decode(c->stack.view(), m, c->d_hull, c->scratch.view(), c->d_has, c->stream), "decode");
+            launch(c, KIND_HULL_DECODE, "decode", [&] {
+                return c->ks->hull_decode(c->stack.view(), m, c->d_hull, c->scratch.view(), c->d_has, c->stream);
+            });
             hlb.assign(static_cast<std::size_t>(c->N) * m, 0);
             hub.assign(static_cast<std::size_t>(c->N) * m, 0);
             download_soa(c, c->scratch, m, hlb.data(), hub.data(), m);
@@ -527,8 +615,10 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
         if(n > c->fin.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one final pass");
         upload_soa(c, c->fin, lb, ub, n, 0);
         if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
-        else hip_check(c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
-                                             static_cast<unsigned>((n + 255) / 256 + 1), c->stream), "final");
+        else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
+            return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
+                                         static_cast<unsigned>((n + 255) / 256 + 1), c->stream);
+        });
         unsigned long long k = get_count(c, c->crit.count);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
@@ -577,6 +667,28 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
         unsigned long long k = get_count(c, c->crit.count);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+    });
+}
+
+int sqd_core_set_profiling(sqd_core* c, int enable) {
+    return guarded([&] {
+        if(!c) throw std::invalid_argument("null core");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        c->profiling = enable != 0;
+        if(c->profiling) c->stats.clear();
+    });
+}
+
+int sqd_core_kernel_stats(const sqd_core* c, sqd_kernel_stat* out, size_t capacity, size_t* count) {
+    return guarded([&] {
+        if(!c || !count) throw std::invalid_argument("null argument");
+        *count = c->stats.size();
+        std::size_t i = 0;
+        for(const auto& kv : c->stats) {
+            if(!out || i >= capacity) break;
+            out[i++] = kv.second;
+        }
     });
 }
 

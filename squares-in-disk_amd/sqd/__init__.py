@@ -40,6 +40,15 @@ class Settings(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("mode", ctypes.c_int32), ("level", ctypes.c_int32),
+                ("launches", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"name": self.name.decode(), "mode": self.mode, "level": self.level, "launches": int(self.launches),
+                "milliseconds": self.milliseconds}
+
+
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _SZ = ctypes.c_size_t
@@ -72,6 +81,8 @@ def lib():
             "sqd_split": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
             "sqd_prove": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _D, _D, _SZ, ctypes.POINTER(_SZ),
                                          ctypes.POINTER(ctypes.c_double)]),
+            "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
+            "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
             "sqd_last_error": (ctypes.c_char_p, []),
         }
@@ -88,8 +99,8 @@ def exported_symbols():
     return ["sqd_settings_default", "sqd_program_from_text", "sqd_program_from_lemma", "sqd_program_destroy",
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
-            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_rounded_op",
-            "sqd_last_error"]
+            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_set_profiling",
+            "sqd_core_kernel_stats", "sqd_rounded_op", "sqd_last_error"]
 
 
 def _check(rc):
@@ -244,6 +255,19 @@ class Core:
         m = min(k.value, critical_capacity)
         return {"counts": c.as_dict(), "crit_lb": cl[:, :m].copy(), "crit_ub": cu[:, :m].copy(),
                 "n_critical": int(k.value), "seconds": secs.value}
+
+
+    def set_profiling(self, enable=True):
+        """Time every launch with HIP events on the core's stream (clears the statistics)."""
+        _check(lib().sqd_core_set_profiling(self._h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        """Per-kernel-kind launches and summed event milliseconds since profiling was enabled."""
+        n = _SZ()
+        _check(lib().sqd_core_kernel_stats(self._h, None, 0, ctypes.byref(n)))
+        arr = (KernelStat * max(1, n.value))()
+        _check(lib().sqd_core_kernel_stats(self._h, arr, n.value, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
 
 def rounded_op(op, a, b, device=0):
