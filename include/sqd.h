@@ -135,8 +135,10 @@ typedef struct sqd_kernel_stat {
     double milliseconds;  /* summed event time over the launches */
 } sqd_kernel_stat;
 
-/* enable != 0 turns profiling on and clears the statistics; 0 turns it off. */
-int sqd_core_set_profiling(sqd_core* core, int enable);
+/* enable != 0 turns profiling on and clears the statistics; 0 turns it off.
+ * With only_mode >= 0 just the level kernel (only_mode, only_level) is timed,
+ * which keeps the event overhead out of the other launches. */
+int sqd_core_set_profiling(sqd_core* core, int enable, int only_mode, int only_level);
 /* Copies up to `capacity` entries (ordered by kind); *count receives the number of kinds. */
 int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t capacity, size_t* count);
 

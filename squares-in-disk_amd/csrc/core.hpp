@@ -201,6 +201,68 @@ __device__ __forceinline__ void store_box(const BoxList& l, unsigned long long i
     }
 }
 
+// ---------------------------------------------------------------- hull join
+// Joins the leaves of lanes with `join` set into their dynamic cuboid's hull
+// record [origin][arg][4]: min order key of lb, max order key of ub, and for
+// bounds equal to zero the first leaf index with its sign bit (the reference
+// keeps the first of equal bounds, so the sign of a zero bound is that of the
+// first leaf in enumeration order).  Called by every lane of the wave.  When all
+// joining lanes share one origin (the usual case: a wave covers consecutive
+// children of one parent) the min/max are reduced across the wave first and one
+// lane issues the atomics.
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+template<int N>
+__device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, unsigned origin, unsigned leaf,
+                                          const double (&lb)[N], const double (&ub)[N]) {
+    const unsigned long long jm = __ballot(join);
+    if(!jm) return;
+    const int first = __ffsll(static_cast<long long>(jm)) - 1;
+    const unsigned o0 = __shfl(origin, first);
+    bool odd = join && origin != o0;
+#pragma unroll
+    for(int a = 0; a < N; ++a) odd = odd || (join && (lb[a] != lb[a] || ub[a] != ub[a]));  // NaN: keep per lane
+    unsigned long long* rec = hull + static_cast<unsigned long long>(join ? origin : o0) * N * 4;
+    if(__ballot(odd) == 0) {
+#pragma unroll
+        for(int a = 0; a < N; ++a) {
+            const double mn = wave_min(join ? lb[a] : __builtin_inf());
+            const double mx = wave_max(join ? ub[a] : -__builtin_inf());
+            if(__lane_id() == first) {
+                atomicMin(rec + a * 4 + 0, order_key(mn));
+                atomicMax(rec + a * 4 + 1, order_key(mx));
+            }
+        }
+    } else if(join) {
+#pragma unroll
+        for(int a = 0; a < N; ++a) {
+            atomicMin(rec + a * 4 + 0, order_key(lb[a]));
+            atomicMax(rec + a * 4 + 1, order_key(ub[a]));
+        }
+    }
+    if(join) {
+#pragma unroll
+        for(int a = 0; a < N; ++a) {
+            if(lb[a] == 0.0) atomicMin(rec + a * 4 + 2, (static_cast<unsigned long long>(leaf) << 1) | (bits(lb[a]) >> 63));
+            if(ub[a] == 0.0) atomicMin(rec + a * 4 + 3, (static_cast<unsigned long long>(leaf) << 1) | (bits(ub[a]) >> 63));
+        }
+    }
+}
+
 // ---------------------------------------------------------------- static levels
 // Level L of the static grid (modes NONFINAL and FINAL).
 template<class P, int MODE, int L>
@@ -224,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
     for(unsigned long long trip = 0; trip < trips; ++trip) {
         const unsigned long long w = first + trip * stride;
         const bool valid = w < total;
-        bool survive = false;
+        bool survive = false, join = false;
         double lb[N], ub[N];
         unsigned origin = 0, leaf = 0;
         if(valid) {
@@ -247,20 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
             } else if(LAST) {
                 ++c_leaf;
                 if(MODE == FINAL) { ++c_crit; survive = true; }
-                else {
-                    unsigned long long* rec = args.hull + static_cast<unsigned long long>(origin) * N * 4;
-#pragma unroll
-                    for(int a = 0; a < N; ++a) {
-                        atomicMin(rec + a * 4 + 0, order_key(lb[a]));
-                        atomicMax(rec + a * 4 + 1, order_key(ub[a]));
-                        if(lb[a] == 0.0) atomicMin(rec + a * 4 + 2, (static_cast<unsigned long long>(leaf) << 1) | (bits(lb[a]) >> 63));
-                        if(ub[a] == 0.0) atomicMin(rec + a * 4 + 3, (static_cast<unsigned long long>(leaf) << 1) | (bits(ub[a]) >> 63));
-                    }
-                }
+                else join = true;
             } else {
                 survive = true;
             }
         }
+        if(LAST && MODE == NONFINAL) hull_join<N>(args.hull, join, origin, leaf, lb, ub);
         if(!LAST || MODE == FINAL) {
             const unsigned long long pos = wave_append(survive, args.out.count);
             if(survive) {
@@ -436,10 +490,18 @@ __global__ void k_initial_box(const double* box_lb, const double* box_ub, const 
     double lb[P::N], ub[P::N];
 #pragma unroll
     for(int a = 0; a < P::N; ++a) { lb[a] = box_lb[a]; ub[a] = box_ub[a]; }
+    // Up to 50 N sweeps over all bounds, each sweep stopping at an empty result.
+    // A sweep that changes nothing leaves every later sweep unchanged too (each
+    // bound application is a function of the box alone), so the loop stops there
+    // with the same box the full count of sweeps would give.
     for(int it = 0; it < 50 * P::N; ++it) {
+        bool changed = false;
         for(int k = 0; k < P::NB; ++k) {
-            if(P::apply(k, lb, ub, s_c) < 0) break;
+            const int r = P::apply(k, lb, ub, s_c);
+            if(r != 0) changed = true;
+            if(r < 0) break;
         }
+        if(!changed) break;
     }
 #pragma unroll
     for(int a = 0; a < P::N; ++a) { out[a] = lb[a]; out[P::N + a] = ub[a]; }

@@ -15,8 +15,14 @@ struct KernelSet {
     const char* key;
     int N, NB, NC, NDYN, NSTAT;
     const double* consts;   // 2*NC doubles: [lb, ub] per constant
-    hipError_t (*static_level)(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s);
-    hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s);
+    // Level launches.  `max_children` bounds the number of children of the
+    // launch (host-known upper bound); `cus` is the device's compute-unit count.
+    // The grid is the kernel's resident capacity (occupancy x CUs), capped by the
+    // blocks the bound needs; a nonzero `grid` overrides it.
+    hipError_t (*static_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
+                               unsigned cus, unsigned grid, hipStream_t s);
+    hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
+                                unsigned cus, unsigned grid, hipStream_t s);
     hipError_t (*hull_finalize)(dev::BoxList items, unsigned long long base, unsigned long long n,
                                 const unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
                                 dev::DeviceCounters* c, unsigned gens, hipStream_t s);
@@ -31,43 +37,66 @@ const KernelSet* find_kernel_set(std::uint64_t fingerprint);
 
 namespace detail {
 
+// Resident blocks per CU of a kernel (queried once per instantiation).
+template<typename K>
+unsigned blocks_per_cu(K kernel) {
+    int n = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, dev::kBlock, 0) != hipSuccess || n < 1) n = 1;
+    return static_cast<unsigned>(n);
+}
+inline unsigned level_grid(unsigned resident, unsigned long long max_children, unsigned grid) {
+    if(grid) return grid;
+    const unsigned long long need = (max_children + dev::kBlock - 1) / dev::kBlock;
+    return static_cast<unsigned>(need < 1 ? 1 : (need < resident ? need : resident));
+}
+
 template<class P, int MODE, int L>
-hipError_t launch_one_static(const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L>), dim3(grid), dim3(dev::kBlock), 0, s, a);
+hipError_t launch_one_static(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
+                             hipStream_t s) {
+    static const unsigned per_cu = blocks_per_cu(dev::k_static_level<P, MODE, L>);
+    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L>), dim3(level_grid(per_cu * cus, max_children, grid)),
+                       dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
 template<class P, int MODE, int J>
-hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_dynamic_level<P, MODE, J>), dim3(grid), dim3(dev::kBlock), 0, s, a);
+hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
+                              hipStream_t s) {
+    static const unsigned per_cu = blocks_per_cu(dev::k_dynamic_level<P, MODE, J>);
+    hipLaunchKernelGGL((dev::k_dynamic_level<P, MODE, J>), dim3(level_grid(per_cu * cus, max_children, grid)),
+                       dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
 template<class P, int MODE, int... L>
-hipError_t launch_static(int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s, std::integer_sequence<int, L...>) {
+hipError_t launch_static(int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus, unsigned grid,
+                         hipStream_t s, std::integer_sequence<int, L...>) {
     hipError_t e = hipErrorInvalidValue;
-    (void)a; (void)grid; (void)s; (void)level;
-    ((level == L ? (e = launch_one_static<P, MODE, L>(a, grid, s), 0) : 0), ...);
+    (void)a; (void)mc; (void)cus; (void)grid; (void)s; (void)level;
+    ((level == L ? (e = launch_one_static<P, MODE, L>(a, mc, cus, grid, s), 0) : 0), ...);
     return e;
 }
 template<class P, int MODE, int... J>
-hipError_t launch_dynamic(int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s, std::integer_sequence<int, J...>) {
+hipError_t launch_dynamic(int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus, unsigned grid,
+                          hipStream_t s, std::integer_sequence<int, J...>) {
     hipError_t e = hipErrorInvalidValue;
-    (void)a; (void)grid; (void)s; (void)level;
-    ((level == J ? (e = launch_one_dynamic<P, MODE, J>(a, grid, s), 0) : 0), ...);
+    (void)a; (void)mc; (void)cus; (void)grid; (void)s; (void)level;
+    ((level == J ? (e = launch_one_dynamic<P, MODE, J>(a, mc, cus, grid, s), 0) : 0), ...);
     return e;
 }
 
 template<class P>
-hipError_t static_level(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+hipError_t static_level(int mode, int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus,
+                        unsigned grid, hipStream_t s) {
     auto seq = std::make_integer_sequence<int, P::NSTAT>{};
-    if(mode == dev::NONFINAL) return launch_static<P, dev::NONFINAL>(level, a, grid, s, seq);
-    if(mode == dev::FINAL) return launch_static<P, dev::FINAL>(level, a, grid, s, seq);
+    if(mode == dev::NONFINAL) return launch_static<P, dev::NONFINAL>(level, a, mc, cus, grid, s, seq);
+    if(mode == dev::FINAL) return launch_static<P, dev::FINAL>(level, a, mc, cus, grid, s, seq);
     return hipErrorInvalidValue;
 }
 template<class P>
-hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned grid, hipStream_t s) {
+hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus,
+                         unsigned grid, hipStream_t s) {
     auto seq = std::make_integer_sequence<int, P::NDYN>{};
-    if(mode == dev::SPLIT) return launch_dynamic<P, dev::SPLIT>(level, a, grid, s, seq);
-    if(mode == dev::INIT) return launch_dynamic<P, dev::INIT>(level, a, grid, s, seq);
+    if(mode == dev::SPLIT) return launch_dynamic<P, dev::SPLIT>(level, a, mc, cus, grid, s, seq);
+    if(mode == dev::INIT) return launch_dynamic<P, dev::INIT>(level, a, mc, cus, grid, s, seq);
     return hipErrorInvalidValue;
 }
 template<class P>

@@ -10,12 +10,12 @@
 // negations away nor move them across the mode switch.
 //
 // Division and square root have no single correctly rounded instruction on
-// CDNA.  They start from the compiler's sequence (accurate to a few ulp in any
-// rounding mode) and are then corrected to the exact directed result with the
-// exact residual a - q*b (resp. a - s*s) computed by one fma (Markstein /
-// Tuckerman correction, Muller et al., Handbook of Floating-Point Arithmetic,
-// ch. 4 and 5).  The residual is exact whenever q is within a few ulp of the
-// quotient and no underflow occurs, so the sign test decides the rounding.
+// CDNA.  They start from an estimate accurate to a few ulp in any rounding mode
+// (reciprocal and reciprocal-square-root estimates refined by Newton /
+// Goldschmidt steps) and are then corrected to the exact directed
+// result from the sign of the residual a - q*b (resp. a - s*s), computed by one
+// fma (Markstein / Tuckerman correction, Muller et al., Handbook of
+// Floating-Point Arithmetic, ch. 4 and 5).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -67,6 +67,12 @@ __device__ __forceinline__ double fms_rd(double a, double b, double c) {
     __asm__ volatile("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(b), "v"(c), "v"(a));
     return r;
 }
+// RU(a*b + c) = -RD(-a*b - c)
+__device__ __forceinline__ double fma_ru(double a, double b, double c) {
+    double r;
+    __asm__ volatile("v_fma_f64 %0, -%1, %2, -%3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return -r;
+}
 
 __device__ __forceinline__ std::uint64_t bits(double x) { return __builtin_bit_cast(std::uint64_t, x); }
 __device__ __forceinline__ double from_bits(std::uint64_t u) { return __builtin_bit_cast(double, u); }
@@ -83,80 +89,144 @@ __device__ __forceinline__ double next_up(double x) {
     return from_bits((u >> 63) ? u - 1 : u + 1);
 }
 
-// Opaque initial approximations (kept out of constant folding).
-__device__ __forceinline__ double approx_div(double a, double b) {
+// IEEE a / b for the special operands (zeros, infinities, NaN).
+__device__ __forceinline__ double special_div(double a, double b) {
     __asm__ volatile("" : "+v"(a), "+v"(b));
     return a / b;
 }
-__device__ __forceinline__ double approx_sqrt(double a) {
+// a / b for |a|, |b| in [2^-501, 2^501]: reciprocal estimate, two Newton steps,
+// and a residual correction whose last rounding is upward, so that the result is
+// RD(a / b) or the next double above it (up to the estimate's tiny error).
+__device__ __forceinline__ double div_estimate(double a, double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return fma_ru(r, y, q);
+}
+// IEEE sqrt for the special inputs (zeros, negatives, infinities, NaN).
+__device__ __forceinline__ double special_sqrt(double a) {
     __asm__ volatile("" : "+v"(a));
     return __builtin_sqrt(a);
 }
 
-// RD(a / b) for normal operands whose quotient is normal (|a|, |b| in [0.5, 1)).
-__device__ inline double rd_div_core(double a, double b) {
-    double q = approx_div(a, b);
+// The directed results below are decided by the sign of an exact residual.
+// For a candidate s of sqrt(m): fma gives RD(m - s*s); a nonzero residual keeps
+// its sign under downward rounding as long as it does not underflow to zero,
+// which the operand ranges guarantee (see the callers).  The candidate q of a / b
+// is judged the same way from RD(a - q*b).
+
+// Slow path: walk q to RD(a / b) (used when the approximation is more than one
+// step off, which the fast path detects).
+__device__ __noinline__ double rd_div_walk(double a, double b, double q) {
     const bool bneg = b < 0.0;
-    // q too large while (a - q b)/b < 0
-    #pragma unroll 1
-    for(int i = 0; i < 4; ++i) {
+    for(int i = 0; i < 64; ++i) {  // q too large: (a - q b) has the sign opposite to b
         double r = fms_rd(a, q, b);
         if(r != 0.0 && ((r < 0.0) != bneg)) q = next_down(q); else break;
     }
-    #pragma unroll 1
-    for(int i = 0; i < 4; ++i) {
+    for(int i = 0; i < 64; ++i) {
         double qn = next_up(q);
         double r = fms_rd(a, qn, b);
         if(r == 0.0 || ((r < 0.0) == bneg)) q = qn; else break;
     }
     return q;
 }
-// RD(a / b).  The operands are normalised to mantissas in [0.5, 1) so that the
-// residual of the correction is exact even for subnormal inputs; the exponent is
-// restored with ldexp, which rounds in the current (downward) mode when the
-// result is subnormal or overflows.  The subnormal grid is a subset of the grid
-// of the scaled quotient, so this second downward rounding equals one rounding.
+// RD(a / b) from a candidate q: one step in the direction the residual points,
+// confirmed by the residual of the neighbour; otherwise the slow walk.
+__device__ __forceinline__ double rd_div_fix(double a, double b, double q) {
+    const bool bneg = b < 0.0;
+    const double r0 = fms_rd(a, q, b);                 // sign(a - q b) = sign(b) * sign(x - q)
+    const bool above = r0 != 0.0 && ((r0 < 0.0) != bneg);   // q > x
+    const double s1 = above ? next_down(q) : next_up(q);
+    const double r1 = fms_rd(a, s1, b);
+    const bool s1_above = r1 != 0.0 && ((r1 < 0.0) != bneg);  // s1 > x
+    // above: s1 < q, RD = s1 when s1 <= x.   below/equal: RD = q when s1 > x, s1 when s1 == x.
+    const bool ok = above ? !s1_above : (s1_above || r1 == 0.0);
+    const double res = above ? s1 : (r1 == 0.0 ? s1 : q);
+    if(__builtin_expect(!ok, 0)) return rd_div_walk(a, b, q);
+    return res;
+}
+// RD(a / b).  Operands with magnitudes in [2^-500, 2^500] keep the quotient and
+// the residual far from underflow and overflow; others are first normalised to
+// mantissas in [0.5, 1) with frexp and the exponent restored with ldexp, which
+// rounds downward in the current mode when the result is subnormal (the
+// subnormal grid is a subset of the normalised quotient's grid, so that second
+// downward rounding equals a single one).
 __device__ inline double rd_div(double a, double b) {
-    if(!is_finite(a) || !is_finite(b) || a == 0.0 || b == 0.0) return approx_div(a, b);  // exact IEEE specials
-    int ea, eb;
-    const double ma = __builtin_frexp(a, &ea);
-    const double mb = __builtin_frexp(b, &eb);
-    return __builtin_ldexp(rd_div_core(ma, mb), ea - eb);
+    if(!is_finite(a) || !is_finite(b) || a == 0.0 || b == 0.0) return special_div(a, b);  // exact IEEE specials
+    const unsigned ea = static_cast<unsigned>(bits(a) >> 52) & 0x7ffu;
+    const unsigned eb = static_cast<unsigned>(bits(b) >> 52) & 0x7ffu;
+    if(ea - 523u <= 1000u && eb - 523u <= 1000u)  // biased exponents in [523, 1523]
+        return rd_div_fix(a, b, div_estimate(a, b));
+    int xa, xb;
+    const double ma = __builtin_frexp(a, &xa);
+    const double mb = __builtin_frexp(b, &xb);
+    return __builtin_ldexp(rd_div_fix(ma, mb, div_estimate(ma, mb)), xa - xb);
 }
 // RU(a / b) = -RD(-a / b)
 __device__ inline double ru_div(double a, double b) { return -rd_div(-a, b); }
 
-// RD(sqrt(a)) for a in [0.5, 2).
-__device__ inline double rd_sqrt_core(double a) {
-    double s = approx_sqrt(a);
-    #pragma unroll 1
-    for(int i = 0; i < 4; ++i) {
-        if(fms_rd(a, s, s) < 0.0) s = next_down(s); else break;
+// sqrt(m) for m in [2^-1000, 2^1024): reciprocal square root estimate, one
+// Goldschmidt step and two residual corrections, the last rounded upward, so
+// that the result is RD(sqrt(m)) or the next double above it.
+__device__ __forceinline__ double sqrt_estimate(double m) {
+    const double y = __builtin_amdgcn_rsq(m);
+    double g = m * y;
+    double h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, m);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, m);
+    return fma_ru(d, h, g);
+}
+__device__ __noinline__ double rd_sqrt_walk(double m, double s) {
+    for(int i = 0; i < 64; ++i) {
+        if(fms_rd(m, s, s) < 0.0) s = next_down(s); else break;
     }
-    #pragma unroll 1
-    for(int i = 0; i < 4; ++i) {
-        double t = next_up(s);
-        if(fms_rd(a, t, t) >= 0.0) s = t; else break;
+    for(int i = 0; i < 64; ++i) {
+        const double t = next_up(s);
+        if(fms_rd(m, t, t) >= 0.0) s = t; else break;
     }
     return s;
 }
-// RD(sqrt(a)): a = m 2^(2k) with m in [0.5, 2), sqrt(a) = sqrt(m) 2^k exactly.
-__device__ inline double rd_sqrt(double a) {
-    if(!(a > 0.0) || !is_finite(a)) return approx_sqrt(a);  // 0, -0, +inf, negative, NaN
-    int e;
-    double m = __builtin_frexp(a, &e);
-    if(e & 1) { m = __builtin_ldexp(m, 1); e -= 1; }
-    return __builtin_ldexp(rd_sqrt_core(m), e / 2);
+// RD(sqrt(m)) from a positive candidate g (as rd_div_fix).
+__device__ __forceinline__ double rd_sqrt_fix(double m, double g) {
+    const double r0 = fms_rd(m, g, g);
+    const bool above = r0 < 0.0;                        // g > sqrt(m)
+    const double s1 = from_bits(bits(g) + (above ? ~0ull : 1ull));   // g > 0: neighbour below / above
+    const double r1 = fms_rd(m, s1, s1);
+    const bool ok = above ? (r1 >= 0.0) : (r1 <= 0.0);
+    const double res = above ? s1 : (r1 == 0.0 ? s1 : g);
+    if(__builtin_expect(!ok, 0)) return rd_sqrt_walk(m, g);
+    return res;
 }
-// RU(sqrt(a))
+// Positive finite m: inputs below 2^-900 are scaled by 2^1000 (an even power,
+// so the root scales exactly by 2^500) to keep the residual from underflowing.
+__device__ __forceinline__ double scale_for_sqrt(double a, bool& tiny) {
+    tiny = a < 0x1p-900;
+    return tiny ? __builtin_ldexp(a, 1000) : a;
+}
+// RD(sqrt(a))
+__device__ inline double rd_sqrt(double a) {
+    if(!(a > 0.0) || !is_finite(a)) return special_sqrt(a);  // 0, -0, +inf, negative, NaN
+    bool tiny;
+    const double m = scale_for_sqrt(a, tiny);
+    const double s = rd_sqrt_fix(m, sqrt_estimate(m));
+    return tiny ? __builtin_ldexp(s, -500) : s;
+}
+// RU(sqrt(a)): RD result, one step up unless the root is exact.
 __device__ inline double ru_sqrt(double a) {
-    if(!(a > 0.0) || !is_finite(a)) return approx_sqrt(a);
-    int e;
-    double m = __builtin_frexp(a, &e);
-    if(e & 1) { m = __builtin_ldexp(m, 1); e -= 1; }
-    double s = rd_sqrt_core(m);
-    if(fms_rd(m, s, s) > 0.0) s = next_up(s);
-    return __builtin_ldexp(s, e / 2);
+    if(!(a > 0.0) || !is_finite(a)) return special_sqrt(a);
+    bool tiny;
+    const double m = scale_for_sqrt(a, tiny);
+    double s = rd_sqrt_fix(m, sqrt_estimate(m));
+    if(fms_rd(m, s, s) != 0.0) s = from_bits(bits(s) + 1ull);
+    return tiny ? __builtin_ldexp(s, -500) : s;
 }
 
 } // namespace dev

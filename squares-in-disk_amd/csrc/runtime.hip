@@ -65,6 +65,20 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned
     for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
         i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) p[i] = v;
 }
+// Entry 0 of a box list (SoA, stride `stride`) from a [lb..., ub...] box; count = 1.
+__global__ void k_box_to_list(const double* box, int n_args, double* lb, double* ub, unsigned* depth,
+                              unsigned long long stride, unsigned long long* count) {
+    const int a = static_cast<int>(threadIdx.x);
+    if(blockIdx.x != 0) return;
+    if(a < n_args) {
+        lb[static_cast<unsigned long long>(a) * stride] = box[a];
+        ub[static_cast<unsigned long long>(a) * stride] = box[n_args + a];
+    }
+    if(a == 0) {
+        if(depth) depth[0] = 0;
+        *count = 1;
+    }
+}
 // hull record initial values: min-key, max-key, zero-sign lb, zero-sign ub
 __global__ void k_init_hull(unsigned long long* p, unsigned long long n_records) {
     for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_records;
@@ -138,7 +152,8 @@ struct sqd_core {
     sqd_settings settings{};
     hipStream_t stream = nullptr;
     int N = 0;
-    unsigned grid = 0;
+    unsigned cus = 0;            // compute units
+    unsigned grid_override = 0;  // level grid from the settings (0: resident capacity)
     double* d_consts = nullptr;
     double* d_box_ct = nullptr;     // 2N: outward compile-time box
     double* d_box_init = nullptr;   // 2N: initial runtime box
@@ -152,6 +167,7 @@ struct sqd_core {
     // profiling: events around each launch, resolved at the next stream sync
     struct Pending { int kind; hipEvent_t a, b; };
     bool profiling = false;
+    int profile_only = -1;           // kind to time, or -1 for every kind
     std::vector<hipEvent_t> ev_pool;
     std::size_t ev_used = 0;
     std::vector<Pending> pending;
@@ -197,12 +213,13 @@ hipEvent_t next_event(sqd_core* c) {
 // launch with two events on the core's stream.
 template<typename F> void launch(sqd_core* c, int kind, const char* what, F&& f) {
     hipEvent_t a = nullptr;
-    if(c->profiling) {
+    const bool timed = c->profiling && (c->profile_only < 0 || c->profile_only == kind);
+    if(timed) {
         a = next_event(c);
         hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
     }
     hip_check(f(), what);
-    if(c->profiling) {
+    if(timed) {
         hipEvent_t b = next_event(c);
         hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
         c->pending.push_back({kind, a, b});
@@ -239,9 +256,11 @@ unsigned long long get_count(sqd_core* c, unsigned long long* p) {
 void reset_counters(sqd_core* c) {
     hip_check(hipMemsetAsync(c->d_counters, 0, sizeof(dev::DeviceCounters), c->stream), "hipMemsetAsync counters");
 }
-sqd_counts read_counters(sqd_core* c) {
+// Reads the counters (and, if given, one more device count in the same sync).
+sqd_counts read_counters(sqd_core* c, const unsigned long long* extra = nullptr, unsigned long long* extra_out = nullptr) {
     dev::DeviceCounters h{};
     hip_check(hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy counters");
+    if(extra) hip_check(hipMemcpyAsync(extra_out, extra, sizeof *extra_out, hipMemcpyDeviceToHost, c->stream), "copy count");
     sync(c);
     if(h.overflow) throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
     sqd_counts r;
@@ -273,14 +292,15 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, unsigned long lon
     return a;
 }
 
-// Static levels over `n` inputs taken from `in` (fixed count when n > 0, else
-// the list's device counter).  NONFINAL joins the leaves into d_hull;
-// FINAL appends critical leaves to `crit_out`.
+// Static levels over the inputs taken from `in`: `n` of them when n > 0, else
+// the list's device counter, which is at most `max_items`.  NONFINAL joins the
+// leaves into d_hull; FINAL appends critical leaves to `crit_out`.
 void run_static(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long base, unsigned long long n,
-                const dev::BoxList& crit_out) {
+                unsigned long long max_items, const dev::BoxList& crit_out) {
     const int S = c->ks->NSTAT;
     dev::BoxList prev = in;
     unsigned long long prev_base = base, prev_fixed = n;
+    unsigned long long items = n ? n : max_items;
     for(int L = 0; L < S; ++L) {
         const bool last = (L == S - 1);
         dev::BoxList out = last ? crit_out : c->lvl[L % 2].view();
@@ -288,18 +308,24 @@ void run_static(sqd_core* c, int mode, const dev::BoxList& in, unsigned long lon
         dev::BoxList src = prev;
         if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
         dev::LevelArgs a = level_args(c, src, prev_base, prev_fixed, out);
-        launch(c, mode * 16 + L, "static level launch", [&] { return c->ks->static_level(mode, L, a, c->grid, c->stream); });
+        const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
+        launch(c, mode * 16 + L, "static level launch",
+               [&] { return c->ks->static_level(mode, L, a, children, c->cus, c->grid_override, c->stream); });
         prev = c->lvl[L % 2].view();
         prev_base = 0;
         prev_fixed = 0;
+        items = std::min<unsigned long long>(children, c->lvl[L % 2].cap);
     }
 }
 
-// Dynamic levels (SPLIT or INIT) from `in` to `out` (appending to its counter).
-void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long n, const dev::BoxList& out) {
+// Dynamic levels (SPLIT or INIT) from `in` (n inputs when n > 0, else the
+// list's counter, at most `max_items`) to `out` (appending to its counter).
+void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long n, unsigned long long max_items,
+                 const dev::BoxList& out) {
     const int D = c->ks->NDYN;
     dev::BoxList prev = in;
     unsigned long long prev_fixed = n;
+    unsigned long long items = n ? n : max_items;
     for(int J = 0; J < D; ++J) {
         const bool last = (J == D - 1);
         dev::BoxList o = last ? out : c->lvl[J % 2].view();
@@ -311,9 +337,13 @@ void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long lo
         src.origin = nullptr;
         src.leaf = nullptr;
         dev::LevelArgs a = level_args(c, src, 0, prev_fixed, o);
-        launch(c, mode * 16 + J, "dynamic level launch", [&] { return c->ks->dynamic_level(mode, J, a, c->grid, c->stream); });
+        const unsigned long long children =
+            items * (mode == dev::INIT ? c->dyn_initial[static_cast<std::size_t>(J)] : c->dyn_subs[static_cast<std::size_t>(J)]);
+        launch(c, mode * 16 + J, "dynamic level launch",
+               [&] { return c->ks->dynamic_level(mode, J, a, children, c->cus, c->grid_override, c->stream); });
         prev = c->lvl[J % 2].view();
         prev_fixed = 0;
+        items = std::min<unsigned long long>(children, c->lvl[J % 2].cap);
     }
 }
 
@@ -343,21 +373,22 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
     sync(c);
 }
 
-void compute_initial_box(sqd_core* c) {
+void launch_initial_box(sqd_core* c) {
     launch(c, KIND_INITIAL_BOX, "initial box", [&] {
         return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, c->stream);
     });
-    sync(c);
 }
 
-// Initial queue onto the (empty) stack; returns its length.
+// Initial queue onto the (empty) stack, all on the device; returns its length.
 unsigned long long build_initial_queue(sqd_core* c) {
-    compute_initial_box(c);
-    std::vector<double> box(2 * static_cast<std::size_t>(c->N));
-    hip_check(hipMemcpy(box.data(), c->d_box_init, box.size() * sizeof(double), hipMemcpyDeviceToHost), "copy box");
-    upload_soa(c, c->scratch, box.data(), box.data() + c->N, 1, 0);
+    launch_initial_box(c);
+    launch(c, KIND_SET_COUNT, "k_box_to_list", [&] {
+        hipLaunchKernelGGL(k_box_to_list, dim3(1), dim3(64), 0, c->stream, c->d_box_init, c->N, c->scratch.lb,
+                           c->scratch.ub, c->scratch.depth, c->scratch.cap, c->scratch.count);
+        return hipGetLastError();
+    });
     set_count(c, c->stack.count, 0);
-    run_dynamic(c, dev::INIT, c->scratch.view(), 1, c->stack.view());
+    run_dynamic(c, dev::INIT, c->scratch.view(), 1, 1, c->stack.view());
     return get_count(c, c->stack.count);
 }
 
@@ -375,7 +406,7 @@ unsigned long long run_round(sqd_core* c, unsigned long long S, unsigned long lo
     dev::BoxList items = c->stack.view();
     if(c->ks->NSTAT > 0) {
         init_hull(c, K);
-        run_static(c, dev::NONFINAL, items, base, K, c->crit.view());
+        run_static(c, dev::NONFINAL, items, base, K, K, c->crit.view());
     }
     set_count(c, c->fin.count, 0);
     set_count(c, c->spl.count, 0);
@@ -385,14 +416,14 @@ unsigned long long run_round(sqd_core* c, unsigned long long S, unsigned long lo
     });
     set_count(c, c->stack.count, base);
     // final pass over hulls at the last generation
-    if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
+    if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, K, c->crit.view());
     else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
         return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters, static_cast<unsigned>((K + 255) / 256),
                                      c->stream);
     });
     // dynamic split of the other hulls, children pushed onto the stack
     dev::BoxList spl = c->spl.view();
-    run_dynamic(c, dev::SPLIT, spl, 0, c->stack.view());
+    run_dynamic(c, dev::SPLIT, spl, 0, K, c->stack.view());
     return get_count(c, c->stack.count);
 }
 
@@ -477,6 +508,7 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->program = program;
         c->ks = program->ks;
         c->N = program->prog.nargs;
+        if(c->N < 1 || c->N > 64) throw std::invalid_argument("programs need 1 to 64 variables");
         if(settings) c->settings = *settings;
         else sqd_settings_default(&c->settings);
         sqd_settings& s = c->settings;
@@ -486,7 +518,8 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         hip_check(hipSetDevice(device), "hipSetDevice");
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        c->grid = s.grid_blocks ? s.grid_blocks : static_cast<unsigned>(prop.multiProcessorCount) * 8u;
+        c->cus = static_cast<unsigned>(prop.multiProcessorCount);
+        c->grid_override = s.grid_blocks;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         const auto& prog = program->prog;
         for(const auto& sp : prog.splits) {
@@ -547,7 +580,8 @@ int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
     return guarded([&] {
         if(!c || !lb || !ub) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        compute_initial_box(c);
+        launch_initial_box(c);
+        sync(c);
         hip_check(hipMemcpy(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
         hip_check(hipMemcpy(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
     });
@@ -584,7 +618,7 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
             upload_soa(c, c->stack, clb.data(), cub.data(), m, 0);
             if(c->ks->NSTAT > 0) {
                 init_hull(c, m);
-                run_static(c, dev::NONFINAL, c->stack.view(), 0, m, c->crit.view());
+                run_static(c, dev::NONFINAL, c->stack.view(), 0, m, m, c->crit.view());
             }
             launch(c, KIND_HULL_DECODE, "decode", [&] {
                 return c->ks->hull_decode(c->stack.view(), m, c->d_hull, c->scratch.view(), c->d_has, c->stream);
@@ -614,7 +648,7 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
         set_count(c, c->crit.count, 0);
         if(n > c->fin.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one final pass");
         upload_soa(c, c->fin, lb, ub, n, 0);
-        if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, c->crit.view());
+        if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, n, c->crit.view());
         else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
             return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
                                          static_cast<unsigned>((n + 255) / 256 + 1), c->stream);
@@ -635,7 +669,7 @@ int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
         if(n > c->spl.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one split");
         upload_soa(c, c->spl, lb, ub, n, 1);
         set_count(c, c->stack.count, 0);
-        run_dynamic(c, dev::SPLIT, c->spl.view(), 0, c->stack.view());
+        run_dynamic(c, dev::SPLIT, c->spl.view(), 0, n, c->stack.view());
         unsigned long long k = get_count(c, c->stack.count);
         if(n_out) *n_out = k;
         if(out_lb && out_ub) download_soa(c, c->stack, std::min<unsigned long long>(k, capacity), out_lb, out_ub, capacity);
@@ -660,22 +694,25 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
             if(K == 0) throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
             S = run_round(c, S, K);
         }
-        sqd_counts r = read_counters(c);
+        unsigned long long k = 0;
+        sqd_counts r = read_counters(c, c->crit.count, &k);
         auto t1 = std::chrono::steady_clock::now();
         if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
         if(counts) *counts = r;
-        unsigned long long k = get_count(c, c->crit.count);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
     });
 }
 
-int sqd_core_set_profiling(sqd_core* c, int enable) {
+int sqd_core_set_profiling(sqd_core* c, int enable, int only_mode, int only_level) {
     return guarded([&] {
         if(!c) throw std::invalid_argument("null core");
+        if(only_mode > 3 || (only_mode >= 0 && (only_level < 0 || only_level > 15)))
+            throw std::invalid_argument("profiling filter: mode 0..3 and level 0..15");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         sync(c);
         c->profiling = enable != 0;
+        c->profile_only = only_mode >= 0 ? only_mode * 16 + only_level : -1;
         if(c->profiling) c->stats.clear();
     });
 }

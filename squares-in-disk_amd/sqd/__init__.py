@@ -81,7 +81,7 @@ def lib():
             "sqd_split": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
             "sqd_prove": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _D, _D, _SZ, ctypes.POINTER(_SZ),
                                          ctypes.POINTER(ctypes.c_double)]),
-            "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
+            "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
             "sqd_last_error": (ctypes.c_char_p, []),
@@ -257,9 +257,12 @@ class Core:
                 "n_critical": int(k.value), "seconds": secs.value}
 
 
-    def set_profiling(self, enable=True):
-        """Time every launch with HIP events on the core's stream (clears the statistics)."""
-        _check(lib().sqd_core_set_profiling(self._h, 1 if enable else 0))
+    def set_profiling(self, enable=True, only=None):
+        """Time launches with HIP events on the core's stream (clears the statistics).
+
+        only: (mode, level) of the one level kernel to time, or None for every launch."""
+        m, l = only if only is not None else (-1, -1)
+        _check(lib().sqd_core_set_profiling(self._h, 1 if enable else 0, m, l))
 
     def kernel_stats(self):
         """Per-kernel-kind launches and summed event milliseconds since profiling was enabled."""
