@@ -148,6 +148,13 @@ int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t cap
  * 4 sqrt): lo = RD(a op b), hi = RU(a op b).  Arithmetic parity checks. */
 int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi);
 
+/* Interval operations on the device (run-time sign decisions, as for operands
+ * without compile-time bounds): n operand pairs of 3 doubles (lb, ub, undefined
+ * flag 0/1), op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 square(a), 6 -a,
+ * 7 max, 8 min, 9 splitter point (a = (lb, ub, -), b = (i, n, -) -> out lb).
+ * Interval parity checks (ivarp/test/interval_*.cpp, splitter.cpp). */
+int sqd_interval_op(int device, int op, size_t n, const double* a, const double* b, double* out);
+
 const char* sqd_last_error(void);
 
 #ifdef __cplusplus

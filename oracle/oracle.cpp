@@ -871,6 +871,38 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
     });
 }
 
+int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b, double* out) {
+    return guarded([&] {
+        if(op < 0 || op > 9) throw std::invalid_argument("unknown interval op");
+        RoundDownGuard g;
+        for(std::uint64_t i = 0; i < n; ++i) {
+            const IV x{a[3 * i], a[3 * i + 1], a[3 * i + 2] != 0.0};
+            const IV y{b[3 * i], b[3 * i + 1], b[3 * i + 2] != 0.0};
+            IV r{0.0, 0.0, false};
+            switch(op) {
+                case 0: r = IV{rd_add(x.lb, y.lb), ru_add(x.ub, y.ub), x.und || y.und}; break;
+                case 1: r = IV{rd_sub(x.lb, y.ub), ru_sub(x.ub, y.lb), x.und || y.und}; break;
+                case 2: r = mul_check_signs(x, y, kUnbounded, kUnbounded); break;
+                case 3: r = interval_div(x, y, kUnbounded, kUnbounded); break;
+                case 4: r = interval_sqrt(x, kUnbounded); break;
+                case 5: r = mul_check_signs(x, x, kUnbounded, kUnbounded); if(r.lb < 0) r.lb = 0; break;
+                case 6: r = IV{-x.ub, -x.lb, x.und}; break;
+                case 7: r = IV{dmax(x.lb, y.lb), dmax(x.ub, y.ub), x.und || y.und}; break;
+                case 8: r = IV{dmin(x.lb, y.lb), dmin(x.ub, y.ub), x.und || y.und}; break;
+                case 9: {
+                    const Splitter sp(x.lb, x.ub, static_cast<int>(y.ub));
+                    r.lb = sp.point(static_cast<int>(y.lb));
+                    break;
+                }
+                default: break;
+            }
+            out[3 * i] = r.lb;
+            out[3 * i + 1] = r.ub;
+            out[3 * i + 2] = r.und ? 1.0 : 0.0;
+        }
+    });
+}
+
 std::uint64_t oracle_flop_count(void) { return g_flops.load(); }
 void oracle_flop_reset(void) {
     g_flops = 0;

@@ -70,6 +70,13 @@ int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* 
 // op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt.  out_lo = RD(a op b), out_hi = RU(a op b).
 int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* out_lo, double* out_hi);
 
+// Interval operations on n operand pairs, 3 doubles per interval (lb, ub,
+// undefined flag 0/1), without compile-time bound information (run-time sign
+// decisions only).  op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (of a), 5 square (of
+// a), 6 negation (of a), 7 max, 8 min, 9 splitter point: a = (lb, ub, -),
+// b = (i, n, -) -> out lb = point i of n.
+int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b, double* out);
+
 // Binary64 arithmetic operations (add/sub/mul/div/sqrt) executed since the last reset.
 std::uint64_t oracle_flop_count(void);
 void oracle_flop_reset(void);

@@ -61,6 +61,7 @@ def lib():
             "oracle_flop_count": (_U64, []),
             "oracle_flop_reset": (None, []),
             "oracle_flop_enable": (None, [ctypes.c_int]),
+            "oracle_interval_op": (ctypes.c_int, [ctypes.c_int, _U64, _D, _D, _D]),
             "oracle_phase_counts": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
         }
         for name, (res, args) in sig.items():
@@ -173,6 +174,19 @@ def rounded_op(op, a, b):
     hi = np.zeros_like(a)
     _check(lib().oracle_rounded_op(code, a.size, _dp(a), _dp(b), _dp(lo), _dp(hi)))
     return lo, hi
+
+
+INTERVAL_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "neg": 6, "max": 7, "min": 8,
+                "split": 9}
+
+
+def interval_op(op, a, b=None):
+    """Interval op on arrays of shape (n, 3) = (lb, ub, undefined); returns (n, 3)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+    b = np.zeros_like(a) if b is None else np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(a)
+    _check(lib().oracle_interval_op(INTERVAL_OPS[op], a.shape[0], _dp(a), _dp(b), _dp(out)))
+    return out
 
 
 def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=None, timeout=None):

@@ -1,10 +1,10 @@
-// Elementwise directed-rounding kernel behind sqd_rounded_op: the device half
-// of the arithmetic parity check (the reference checks device == host bit for
+// Elementwise kernels behind sqd_rounded_op and sqd_interval_op: the device half
+// of the arithmetic parity checks (the reference checks device == host bit for
 // bit in ivarp/test/cuda/intervals.cu:127-181).
 #include <hip/hip_runtime.h>
 #include <string>
 #include "sqd.h"
-#include "rounding.hpp"
+#include "core.hpp"
 
 namespace {
 __global__ void k_rounded_op(int op, unsigned long long n, const double* a, const double* b, double* lo, double* hi) {
@@ -26,7 +26,57 @@ __global__ void k_rounded_op(int op, unsigned long long n, const double* a, cons
         hi[i] = h;
     }
 }
-thread_local std::string t_err;
+__global__ void k_interval_op(int op, unsigned long long n, const double* a, const double* b, double* out) {
+    using namespace sqd::dev;
+    enter_round_down();
+    for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+        i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        const IV x{a[3 * i], a[3 * i + 1], a[3 * i + 2] != 0.0};
+        const IV y{b[3 * i], b[3 * i + 1], b[3 * i + 2] != 0.0};
+        IV r{0.0, 0.0, false};
+        switch(op) {
+            case 0: r = iv_add(x, y); break;
+            case 1: r = iv_sub(x, y); break;
+            case 2: r = iv_mul<R, R, R, R, false>(x, y); break;
+            case 3: r = iv_div<R, R, R, R, false>(x, y); break;
+            case 4: r = iv_sqrt<R>(x); break;
+            case 5: r = iv_square(x); break;
+            case 6: r = iv_neg(x); break;
+            case 7: r = iv_max(x, y); break;
+            case 8: r = iv_min(x, y); break;
+            case 9: {
+                const int cnt = static_cast<int>(y.ub);
+                r.lb = split_point(x.lb, x.ub, split_width(x.lb, x.ub, cnt), static_cast<int>(y.lb), cnt);
+                break;
+            }
+            default: break;
+        }
+        out[3 * i] = r.lb;
+        out[3 * i + 1] = r.ub;
+        out[3 * i + 2] = r.und ? 1.0 : 0.0;
+    }
+}
+}
+
+extern "C" int sqd_interval_op(int device, int op, size_t n, const double* a, const double* b, double* out) {
+    if(op < 0 || op > 9 || !a || !b || !out) return SQD_ERR_ARGUMENT;
+    if(n == 0) return SQD_OK;
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    const size_t bytes = 3 * n * sizeof(double);
+    hipError_t e = hipSetDevice(device);
+    if(e == hipSuccess) e = hipMalloc(&da, bytes);
+    if(e == hipSuccess) e = hipMalloc(&db, bytes);
+    if(e == hipSuccess) e = hipMalloc(&dout, bytes);
+    if(e == hipSuccess) e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice);
+    if(e == hipSuccess) e = hipMemcpy(db, b, bytes, hipMemcpyHostToDevice);
+    if(e == hipSuccess) {
+        unsigned grid = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_interval_op, dim3(grid), dim3(256), 0, 0, op, static_cast<unsigned long long>(n), da, db, dout);
+        e = hipGetLastError();
+    }
+    if(e == hipSuccess) e = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);
+    for(double* p : {da, db, dout}) if(p) (void)hipFree(p);
+    return e == hipSuccess ? SQD_OK : SQD_ERR_DEVICE;
 }
 
 extern "C" int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi) {

@@ -84,6 +84,7 @@ def lib():
             "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
+            "sqd_interval_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
             "sqd_last_error": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -100,7 +101,7 @@ def exported_symbols():
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_set_profiling",
-            "sqd_core_kernel_stats", "sqd_rounded_op", "sqd_last_error"]
+            "sqd_core_kernel_stats", "sqd_rounded_op", "sqd_interval_op", "sqd_last_error"]
 
 
 def _check(rc):
@@ -282,3 +283,16 @@ def rounded_op(op, a, b, device=0):
     hi = np.zeros_like(a)
     _check(lib().sqd_rounded_op(device, code, a.size, _dptr(a), _dptr(b), _dptr(lo), _dptr(hi)))
     return lo, hi
+
+
+INTERVAL_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "neg": 6, "max": 7, "min": 8,
+                "split": 9}
+
+
+def interval_op(op, a, b=None, device=0):
+    """Device interval op on arrays of shape (n, 3) = (lb, ub, undefined); returns (n, 3)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+    b = np.zeros_like(a) if b is None else np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(a)
+    _check(lib().sqd_interval_op(device, INTERVAL_OPS[op], a.shape[0], _dptr(a), _dptr(b), _dptr(out)))
+    return out

@@ -1,0 +1,172 @@
+"""Interval arithmetic of the CPU oracle against the reference's known-answer
+interval tests (tests/golden/interval_kat.json, transcribed from
+ivarp/test/interval_{addsub,mul,div,sqrt}.cpp and splitter.cpp), against exact
+rational arithmetic (directed rounding of + - * / sqrt is fixed by IEEE 754),
+and against random rational points in the style of the reference's
+random_test_binary_operator (ivarp/test/test_util.hpp:114-198)."""
+import json
+import math
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pyoracle
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "interval_kat.json").read_text())
+INF = float("inf")
+
+
+def dec(v):
+    return INF if v == "inf" else (-INF if v == "-inf" else float.fromhex(v))
+
+
+def iv(t):
+    return [dec(t[0]), dec(t[1]), float(t[2])]
+
+
+@pytest.mark.parametrize("case", GOLDEN["cases"], ids=lambda c: c["op"] + "@" + c["src"].split("/")[-1])
+def test_oracle_interval_known_answers(case):
+    got = pyoracle.interval_op(case["op"], [iv(case["a"])], [iv(case["b"])])[0]
+    exp = iv(case["expect"])
+    assert got[0] == exp[0] and got[1] == exp[1] and bool(got[2]) == bool(exp[2]), (got, exp)
+
+
+def rd(fr):
+    """Largest double <= the rational fr (finite range)."""
+    d = float(fr)
+    return math.nextafter(d, -INF) if Fraction(d) > fr else d
+
+
+def ru(fr):
+    d = float(fr)
+    return math.nextafter(d, INF) if Fraction(d) < fr else d
+
+
+def rd_sqrt_exact(x):
+    """Largest double s with s*s <= x (x >= 0 a double)."""
+    s = math.sqrt(x)
+    fx = Fraction(x)
+    while Fraction(s) ** 2 > fx:
+        s = math.nextafter(s, -INF)
+    while Fraction(math.nextafter(s, INF)) ** 2 <= fx:
+        s = math.nextafter(s, INF)
+    return s
+
+
+def _random_doubles(rng, n):
+    m = rng.uniform(0.5, 1.0, n) * np.exp2(rng.integers(-60, 60, n).astype(float))
+    return m * rng.choice([-1.0, 1.0], n)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div", "sqrt"])
+def test_oracle_directed_rounding_exact(op):
+    rng = np.random.default_rng(11 + len(op))
+    a = _random_doubles(rng, 3000)
+    b = _random_doubles(rng, 3000)
+    if op == "sqrt":
+        a = np.abs(a)
+    lo, hi = pyoracle.rounded_op(op, a, b)
+    for x, y, l, h in zip(a, b, lo, hi):
+        if op == "sqrt":
+            s = rd_sqrt_exact(float(x))
+            exp_l, exp_h = s, (s if Fraction(s) ** 2 == Fraction(float(x)) else math.nextafter(s, INF))
+        else:
+            fx, fy = Fraction(float(x)), Fraction(float(y))
+            e = {"add": fx + fy, "sub": fx - fy, "mul": fx * fy, "div": fx / fy}[op]
+            exp_l, exp_h = rd(e), ru(e)
+        assert (l, h) == (exp_l, exp_h), (op, x, y)
+
+
+def random_interval(rng, rng_lb, rng_ub, widths):
+    wl, wu = widths[rng.integers(len(widths))]
+    w = rng.uniform(wl, wu)
+    lb = rng.uniform(rng_lb, max(rng_lb, rng_ub - w))
+    return [lb, lb + w, 0.0]
+
+
+def random_point(rng, lb, ub):
+    t = Fraction(int(rng.integers(0, 1 << 30)), 1 << 30)
+    return Fraction(lb) + (Fraction(ub) - Fraction(lb)) * t
+
+
+def enclosing(fr):
+    return [rd(fr), ru(fr), 0.0]
+
+
+WIDTHS = [(0, 0.1), (0, 0.5), (0.1, 0.3), (0.5, 2.5), (0.5, 5.0), (5.0, 20.0), (20.0, 50.0)]
+RANGES = {  # ivarp/test/interval_mul.cpp:74-78, interval_div.cpp:69-80, interval_addsub.cpp:88
+    "add": ([(-100, 100)], [(-100, 100)]),
+    "sub": ([(-100, 100)], [(-100, 100)]),
+    "mul": ([(-100, 100), (-1000, -900), (900, 1000)], [(-100, 100), (-1000, -900), (900, 1000)]),
+    "div": ([(-1000, -900), (900, 1000), (-0.9, -0.2), (0.2, 0.9), (-1000, 1000), (-100, 100), (-0.2, 0.2)],
+            [(-1000, -900), (900, 1000), (-0.9, -0.2), (0.2, 0.9)]),
+}
+
+
+def exact(op, p, q):
+    return {"add": p + q, "sub": p - q, "mul": p * q, "div": p / q}[op]
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_oracle_interval_random_enclosure(op):
+    rng = np.random.default_rng(20261202 + len(op))
+    r1, r2 = RANGES[op]
+    A, B = [], []
+    for _ in range(400):
+        A.append(random_interval(rng, *r1[rng.integers(len(r1))], WIDTHS))
+        B.append(random_interval(rng, *r2[rng.integers(len(r2))], WIDTHS))
+    # pairs of random points and their 3/11, 5/7 scaled versions on 1-ulp intervals
+    P = [(random_point(rng, a[0], a[1]), random_point(rng, b[0], b[1])) for a, b in zip(A, B)]
+    Ac = [enclosing(p * 3 / 11) for p, _ in P]
+    Bc = [enclosing(q * 5 / 7) for _, q in P]
+    out = pyoracle.interval_op(op, A, B)
+    outc = pyoracle.interval_op(op, Ac, Bc)
+    for (p, q), r, rc in zip(P, out, outc):
+        if op == "div" and q == 0:
+            continue
+        e = exact(op, p, q)
+        assert (r[0] == -INF or Fraction(r[0]) <= e) and (r[1] == INF or e <= Fraction(r[1]))
+        ec = exact(op, p * 3 / 11, q * 5 / 7)
+        assert (rc[0] == -INF or Fraction(rc[0]) <= ec) and (rc[1] == INF or ec <= Fraction(rc[1]))
+        assert r[0] <= r[1]
+
+
+def test_oracle_interval_sqrt_points_and_random():
+    # ivarp/test/interval_sqrt.cpp:53-69: point intervals of non-squares give adjacent bounds
+    for n in GOLDEN["sqrt_points"]["values"]:
+        r = pyoracle.interval_op("sqrt", [[n, n, 0.0]])[0]
+        assert r[0] < r[1] and math.nextafter(r[0], INF) == r[1] and r[2] == 0.0
+        sq = pyoracle.interval_op("square", [r])[0]
+        assert sq[0] < n < sq[1]
+        assert Fraction(r[0]) ** 2 < n < Fraction(r[1]) ** 2
+    # interval_sqrt.cpp:72-92: random intervals in [0, 100]
+    rng = np.random.default_rng(5)
+    widths = [(0.0, 0.001), (0.0, 0.1), (0.1, 0.5), (0.4, 0.8), (1.0, 2.0), (2.0, 50.0)]
+    X = [random_interval(rng, 0, 100, widths) for _ in range(2000)]
+    R = pyoracle.interval_op("sqrt", X)
+    S = pyoracle.interval_op("square", R)
+    for x, r, s in zip(X, R, S):
+        assert r[2] == 0.0 and r[0] >= 0
+        assert s[0] <= x[0] and s[1] >= x[1]
+        assert Fraction(r[0]) ** 2 <= Fraction(x[0]) and Fraction(r[1]) ** 2 >= Fraction(x[1])
+
+
+def test_oracle_splitter():
+    # ivarp/test/splitter.cpp:27-52, and the round-down formula w = RD(RD(ub - lb) / n),
+    # p_i = RD(lb + RD(i * w)), p_n = ub (ivarp/include/ivarp/splitter.hpp:44-59)
+    n = GOLDEN["splitter"]["n"]
+    lo, hi = GOLDEN["splitter"]["range"]
+    a = [[lo, hi, 0.0]] * (n + 1)
+    b = [[i, n, 0.0] for i in range(n + 1)]
+    pts = pyoracle.interval_op("split", a, b)[:, 0]
+    assert pts[0] == 0 and pts[n - 1] < 11 and pts[n] == 11
+    assert np.all(np.diff(pts) > 0)
+    w = rd(Fraction(rd(Fraction(hi) - Fraction(lo))) / n)
+    for i in range(n):
+        assert pts[i] == rd(Fraction(lo) + Fraction(rd(Fraction(i) * Fraction(w))))
+    # an interval whose width is not representable: the grid still tiles it exactly
+    a = [[0.1, 1.7, 0.0]] * 9
+    pts = pyoracle.interval_op("split", a, [[i, 8, 0.0] for i in range(9)])[:, 0]
+    assert pts[0] == 0.1 and pts[8] == 1.7 and np.all(np.diff(pts) > 0)
