@@ -58,7 +58,8 @@ typedef struct sqd_settings {
     uint64_t frontier_capacity;    /* cuboids the device frontier can hold; default 1<<23 */
     uint64_t level_bytes;          /* bytes per level buffer; default 1<<30 */
     uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
-    uint32_t grid_blocks;          /* blocks per level launch; default 8 per CU */
+    uint32_t grid_blocks;          /* blocks per level launch; default: the kernel's resident
+                                      capacity (occupancy x CUs), capped by the work */
     uint32_t reserved;
 } sqd_settings;
 
