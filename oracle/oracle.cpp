@@ -393,12 +393,61 @@ struct Evaluator {
             default: throw std::runtime_error("unsupported node");
         }
     }
+    // Demand-driven evaluation, memoized per evaluation (one bound or one
+    // constraint on one box).  if_then_else evaluates only the branches its
+    // condition allows (math_fn/eval/if_then_else.hpp:64-82); || and && stop at a
+    // definitely true / not possibly true operand (predicate_eval/n_ary.hpp:63-135).
+    // The values equal those of evaluating every node; the operation count is
+    // that of the lazy evaluation.
+    std::vector<std::uint32_t> stamp = std::vector<std::uint32_t>(p.nodes.size(), 0);
+    std::uint32_t epoch = 0;
+    void begin() {
+        if(++epoch == 0) { std::fill(stamp.begin(), stamp.end(), 0u); epoch = 1; }
+    }
+    void need(int id, const Cuboid& c) {
+        const std::size_t i = static_cast<std::size_t>(id);
+        if(stamp[i] == epoch) return;
+        const PNode& n = p.nodes[i];
+        auto kid = [&](std::size_t j) { return n.kids[j]; };
+        switch(n.op) {
+            case C_ITE: {
+                need(kid(0), c);
+                const IB cnd = pb[static_cast<std::size_t>(kid(0))];
+                if(cnd.poss) need(kid(1), c);
+                if(!cnd.def) need(kid(2), c);
+                eval_node(id, c);   // reads only the branches the condition selects
+                break;
+            }
+            case C_OR: case C_AND: {
+                const bool is_or = n.op == C_OR;
+                need(kid(0), c);
+                IB acc = pb[static_cast<std::size_t>(kid(0))];
+                for(std::size_t j = 1; j < n.kids.size(); ++j) {
+                    if(is_or ? acc.def : !acc.poss) break;
+                    need(kid(j), c);
+                    const IB x = pb[static_cast<std::size_t>(kid(j))];
+                    if(is_or) { acc.def = acc.def || x.def; acc.poss = acc.poss || x.poss; }
+                    else { acc.def = acc.def && x.def; acc.poss = acc.poss && x.poss; }
+                }
+                pb[i] = acc;
+                break;
+            }
+            default:
+                for(int k : n.kids) need(k, c);
+                eval_node(id, c);
+                break;
+        }
+        stamp[i] = epoch;
+    }
     const IV& eval_bound(int k, const Cuboid& c) {
-        for(int id : p.bound_order[static_cast<std::size_t>(k)]) eval_node(id, c);
-        return v[static_cast<std::size_t>(p.bounds[static_cast<std::size_t>(k)].expr)];
+        begin();
+        const int e = p.bounds[static_cast<std::size_t>(k)].expr;
+        need(e, c);
+        return v[static_cast<std::size_t>(e)];
     }
     bool possibly(int pred, const Cuboid& c) {
-        for(int id : p.pred_order[static_cast<std::size_t>(pred)]) eval_node(id, c);
+        begin();
+        need(pred, c);
         return pb[static_cast<std::size_t>(pred)].poss;
     }
 };

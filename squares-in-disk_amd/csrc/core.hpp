@@ -34,6 +34,17 @@ namespace dev {
 
 constexpr int kBlock = 256;
 
+// Register budget of the level kernels, as a minimum of resident waves per SIMD
+// (0: the compiler's choice).
+#ifndef SQD_LEVEL_WAVES_PER_EU
+#define SQD_LEVEL_WAVES_PER_EU 0
+#endif
+#if SQD_LEVEL_WAVES_PER_EU > 0
+#define SQD_LEVEL_ATTR __attribute__((amdgpu_waves_per_eu(SQD_LEVEL_WAVES_PER_EU)))
+#else
+#define SQD_LEVEL_ATTR
+#endif
+
 enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
 
 struct DeviceCounters {
@@ -55,12 +66,15 @@ struct BoxList {
     unsigned long long* count;  // device counter of valid entries
 };
 
+// Arguments of a level kernel.  The input range lives in device memory (the
+// round's pop position and size, or a list's counter), so a whole round can be
+// enqueued without the host knowing its size.
 struct LevelArgs {
     BoxList in;
     BoxList out;
-    unsigned long long in_base;        // first input entry
-    unsigned long long in_fixed_count; // when nonzero: number of inputs (count pointer ignored)
-    unsigned long long* hull;          // NONFINAL last level: [origin][arg][4] keys
+    const unsigned long long* in_base;   // first input entry (nullptr: 0)
+    const unsigned long long* in_count;  // number of inputs
+    unsigned long long* hull;            // NONFINAL last level: [origin][arg][4] keys
     DeviceCounters* counters;
     const double* consts;
     unsigned generations;
@@ -180,9 +194,7 @@ __device__ __forceinline__ void load_consts(double* dst, const double* src) {
     __syncthreads();
 }
 
-__device__ __forceinline__ unsigned long long item_count(const BoxList& in, unsigned long long fixed) {
-    return fixed ? fixed : *in.count;
-}
+__device__ __forceinline__ unsigned long long input_base(const LevelArgs& a) { return a.in_base ? *a.in_base : 0ull; }
 
 template<class P>
 __device__ __forceinline__ void load_box(const BoxList& l, unsigned long long i, double (&lb)[P::N], double (&ub)[P::N]) {
@@ -266,7 +278,7 @@ __device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, u
 // ---------------------------------------------------------------- static levels
 // Level L of the static grid (modes NONFINAL and FINAL).
 template<class P, int MODE, int L>
-__global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs args) {
     constexpr int N = P::N;
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
@@ -275,7 +287,8 @@ __global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
     __shared__ PropScratch<P> s_prop;
     load_consts<P>(s_c, args.consts);
     enter_round_down();
-    const unsigned long long n_items = item_count(args.in, args.in_fixed_count);
+    const unsigned long long n_items = *args.in_count;
+    const unsigned long long in_base = input_base(args);
     const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
@@ -292,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
         if(valid) {
             const unsigned long long item = w / SUBS;
             const int i = static_cast<int>(w % SUBS);
-            const unsigned long long src = args.in_base + item;
+            const unsigned long long src = in_base + item;
             load_box<P>(args.in, src, lb, ub);
             if(args.in.origin) { origin = args.in.origin[src]; leaf = args.in.leaf[src]; }
             else { origin = static_cast<unsigned>(item); leaf = 0; }
@@ -337,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_static_level(LevelArgs args) {
 // initial queue).  The last level keeps children that possibly satisfy every
 // constraint and appends them to the output (the cuboid stack).
 template<class P, int MODE, int J>
-__global__ __launch_bounds__(kBlock) void k_dynamic_level(LevelArgs args) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelArgs args) {
     constexpr int N = P::N;
     constexpr int ARG = P::dyn_arg(J);
     constexpr int SUBS = MODE == INIT ? P::dyn_initial(J) : P::dyn_subs(J);
@@ -347,7 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_dynamic_level(LevelArgs args) {
     __shared__ PropScratch<P> s_prop;
     load_consts<P>(s_c, args.consts);
     enter_round_down();
-    const unsigned long long n_items = item_count(args.in, args.in_fixed_count);
+    const unsigned long long n_items = *args.in_count;
+    const unsigned long long in_base = input_base(args);
     const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_dynamic_level(LevelArgs args) {
         if(w < total) {
             const unsigned long long item = w / SUBS;
             const int i = static_cast<int>(w % SUBS);
-            const unsigned long long src = args.in_base + item;
+            const unsigned long long src = in_base + item;
             load_box<P>(args.in, src, lb, ub);
             depth = args.in.depth ? args.in.depth[src] : 0u;
             ++c_cub;
@@ -395,11 +409,28 @@ __global__ __launch_bounds__(kBlock) void k_dynamic_level(LevelArgs args) {
 // Turn the hull records of one round into boxes.  Hulls at depth >= generations
 // go to `fin` (final pass), the others to `spl` with depth + 1 (dynamic split).
 // Without static variables the popped cuboid itself is the only leaf.
+// Hull records start as (min key, max key, zero-sign keys) = (~0, 0, ~0, ~0)
+// and are put back to that state by the kernel that consumes them.
+__device__ __forceinline__ void clear_hull_record(unsigned long long* rec, int n_args) {
+    for(int a = 0; a < n_args; ++a) {
+        rec[a * 4 + 0] = ~0ull;
+        rec[a * 4 + 1] = 0ull;
+        rec[a * 4 + 2] = ~0ull;
+        rec[a * 4 + 3] = ~0ull;
+    }
+}
+
+// Turn the hull records of one round into boxes.  Hulls at depth >= generations
+// go to `fin` (final pass), the others to `spl` with depth + 1 (dynamic split).
+// Without static variables the popped cuboid itself is the only leaf.  The
+// round's items are [*base, *base + *count) of `items`.
 template<class P>
-__global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, unsigned long long base, unsigned long long n,
-                                                          const unsigned long long* hull, BoxList fin, BoxList spl,
-                                                          DeviceCounters* counters, unsigned generations) {
+__global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, const unsigned long long* base_ptr,
+                                                          const unsigned long long* count_ptr, unsigned long long* hull,
+                                                          BoxList fin, BoxList spl, DeviceCounters* counters,
+                                                          unsigned generations) {
     constexpr int N = P::N;
+    const unsigned long long n = *count_ptr, base = *base_ptr;
     const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool have = false;
     double lb[N], ub[N];
@@ -410,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, unsigne
             load_box<P>(items, base + d, lb, ub);
             have = true;
         } else {
-            const unsigned long long* rec = hull + d * N * 4;
+            unsigned long long* rec = hull + d * N * 4;
             have = rec[0] != ~0ull;
             if(have) {
 #pragma unroll
@@ -420,6 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, unsigne
                     if(lb[a] == 0.0) lb[a] = (rec[a * 4 + 2] & 1ull) ? -0.0 : 0.0;
                     if(ub[a] == 0.0) ub[a] = (rec[a * 4 + 3] & 1ull) ? -0.0 : 0.0;
                 }
+                clear_hull_record(rec, N);
             }
         }
     }
@@ -441,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, unsigne
 // Decode hull records in input order (sqd_nonfinal): entry d of `out` holds the
 // hull of input d, has[d] = 0 when no leaf survived.
 template<class P>
-__global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned long long n, const unsigned long long* hull,
+__global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned long long n, unsigned long long* hull,
                                                         BoxList out, unsigned char* has) {
     constexpr int N = P::N;
     const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -452,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned 
         load_box<P>(items, d, lb, ub);
         ok = true;
     } else {
-        const unsigned long long* rec = hull + d * N * 4;
+        unsigned long long* rec = hull + d * N * 4;
         ok = rec[0] != ~0ull;
 #pragma unroll
         for(int a = 0; a < N; ++a) {
@@ -461,6 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned 
             if(ok && lb[a] == 0.0) lb[a] = (rec[a * 4 + 2] & 1ull) ? -0.0 : 0.0;
             if(ok && ub[a] == 0.0) ub[a] = (rec[a * 4 + 3] & 1ull) ? -0.0 : 0.0;
         }
+        if(ok) clear_hull_record(rec, N);
     }
     store_box<P>(out, d, lb, ub);
     has[d] = ok ? 1 : 0;
@@ -481,8 +514,11 @@ __global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList 
 }
 
 // Initial runtime box: outward compile-time box, then 50*N rounds of all bounds.
+// The box goes to `out` (N lb, N ub) and, when list.lb is set, to entry 0 of
+// `list` (depth 0) with the list's count set to 1.
 template<class P>
-__global__ void k_initial_box(const double* box_lb, const double* box_ub, const double* consts, double* out) {
+__global__ void k_initial_box(const double* box_lb, const double* box_ub, const double* consts, double* out,
+                              BoxList list) {
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
     load_consts<P>(s_c, consts);
     enter_round_down();
@@ -505,6 +541,11 @@ __global__ void k_initial_box(const double* box_lb, const double* box_ub, const 
     }
 #pragma unroll
     for(int a = 0; a < P::N; ++a) { out[a] = lb[a]; out[P::N + a] = ub[a]; }
+    if(list.lb) {
+        store_box<P>(list, 0, lb, ub);
+        if(list.depth) list.depth[0] = 0;
+        *list.count = 1;
+    }
 }
 
 } // namespace dev

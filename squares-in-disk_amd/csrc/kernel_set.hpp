@@ -23,12 +23,14 @@ struct KernelSet {
                                unsigned cus, unsigned grid, hipStream_t s);
     hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
                                 unsigned cus, unsigned grid, hipStream_t s);
-    hipError_t (*hull_finalize)(dev::BoxList items, unsigned long long base, unsigned long long n,
-                                const unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
-                                dev::DeviceCounters* c, unsigned gens, hipStream_t s);
+    // Hulls of the round's items [*base, *base + *count) (at most max_n of them).
+    hipError_t (*hull_finalize)(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
+                                unsigned long long max_n, unsigned long long* hull, dev::BoxList fin,
+                                dev::BoxList spl, dev::DeviceCounters* c, unsigned gens, hipStream_t s);
     hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s);
-    hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out, hipStream_t s);
-    hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, const unsigned long long* hull,
+    hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out,
+                              dev::BoxList list, hipStream_t s);
+    hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
                               dev::BoxList out, unsigned char* has, hipStream_t s);
 };
 
@@ -100,11 +102,13 @@ hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned 
     return hipErrorInvalidValue;
 }
 template<class P>
-hipError_t hull_finalize(dev::BoxList items, unsigned long long base, unsigned long long n, const unsigned long long* hull,
-                         dev::BoxList fin, dev::BoxList spl, dev::DeviceCounters* c, unsigned gens, hipStream_t s) {
-    if(n == 0) return hipSuccess;
-    unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);
-    hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, base, n, hull, fin, spl, c, gens);
+hipError_t hull_finalize(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
+                         unsigned long long max_n, unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
+                         dev::DeviceCounters* c, unsigned gens, hipStream_t s) {
+    if(max_n == 0) return hipSuccess;
+    unsigned grid = static_cast<unsigned>((max_n + dev::kBlock - 1) / dev::kBlock);
+    hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, base, count, hull, fin,
+                       spl, c, gens);
     return hipGetLastError();
 }
 template<class P>
@@ -113,13 +117,14 @@ hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, dev::DeviceCounter
     return hipGetLastError();
 }
 template<class P>
-hipError_t initial_box(const double* lb, const double* ub, const double* consts, double* out, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out);
+hipError_t initial_box(const double* lb, const double* ub, const double* consts, double* out, dev::BoxList list,
+                       hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out, list);
     return hipGetLastError();
 }
 
 template<class P>
-hipError_t hull_decode(dev::BoxList items, unsigned long long n, const unsigned long long* hull, dev::BoxList out,
+hipError_t hull_decode(dev::BoxList items, unsigned long long n, unsigned long long* hull, dev::BoxList out,
                        unsigned char* has, hipStream_t s) {
     if(n == 0) return hipSuccess;
     unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);

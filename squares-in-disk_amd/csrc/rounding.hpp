@@ -4,10 +4,10 @@
 // round-toward-minus-infinity for f32 and f64 (enter_round_down) and computes
 //   lower bounds as RD(x op y)      (one VALU instruction)
 //   upper bounds as -RD((-x) op y)  (= RU(x op y); the negation is a free source modifier)
-// exactly like the reference's SSE code (ivarp/include/ivarp/number/float_interval_addsub.hpp,
-// float_interval_mul.hpp).  The add/mul/fma instructions are written as inline
-// assembly so the compiler, which assumes round-to-nearest, can neither fold the
-// negations away nor move them across the mode switch.
+// like the reference's SSE code (ivarp/include/ivarp/number/float_interval_addsub.hpp,
+// float_interval_mul.hpp).  The mode switch is a volatile asm statement with a
+// memory clobber at kernel entry, before any load: every floating-point value of
+// the kernel derives from memory read after it.
 //
 // Division and square root have no single correctly rounded instruction on
 // CDNA.  They start from an estimate accurate to a few ulp in any rounding mode
@@ -28,51 +28,29 @@ __device__ __forceinline__ void enter_round_down() {
     __asm__ volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 4), 10" ::: "memory");
 }
 
-__device__ __forceinline__ double rd_add(double a, double b) {
-    double r;
-    __asm__ volatile("v_add_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
+// The operations below are plain IEEE operations evaluated in the wave's
+// round-down mode.  The compiler assumes round-to-nearest, so they are written
+// in forms that no transformation valid under round-to-nearest can change:
+// moving a negation across an add, a sub or an fma changes the sign of an exact
+// zero and is only done by the compiler with no-signed-zeros (never enabled),
+// while a negation *is* moved across a plain multiply.  Products are therefore
+// fma(a, b, +0.0), which equals RD(a * b) in round-down mode including the sign
+// of zero (-0 + +0 = -0 when rounding down) and cannot be rewritten as a
+// multiply without no-signed-zeros.  The negations end up as free source
+// modifiers or sign-bit flips.
+__device__ __forceinline__ double rd_add(double a, double b) { return a + b; }
 // RU(a + b) = -RD(-a - b)
-__device__ __forceinline__ double ru_add(double a, double b) {
-    double r;
-    __asm__ volatile("v_add_f64 %0, -%1, -%2" : "=v"(r) : "v"(a), "v"(b));
-    return -r;
-}
-__device__ __forceinline__ double rd_sub(double a, double b) {
-    double r;
-    __asm__ volatile("v_add_f64 %0, %1, -%2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-// RU(a - b) = -RD(-a + b)
-__device__ __forceinline__ double ru_sub(double a, double b) {
-    double r;
-    __asm__ volatile("v_add_f64 %0, -%1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return -r;
-}
-__device__ __forceinline__ double rd_mul(double a, double b) {
-    double r;
-    __asm__ volatile("v_mul_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-// RU(a * b) = -RD((-a) * b)
-__device__ __forceinline__ double ru_mul(double a, double b) {
-    double r;
-    __asm__ volatile("v_mul_f64 %0, -%1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return -r;
-}
+__device__ __forceinline__ double ru_add(double a, double b) { return -((-a) - b); }
+__device__ __forceinline__ double rd_sub(double a, double b) { return a - b; }
+// RU(a - b) = -RD(b - a)
+__device__ __forceinline__ double ru_sub(double a, double b) { return -(b - a); }
+__device__ __forceinline__ double rd_mul(double a, double b) { return __builtin_fma(a, b, 0.0); }
+// RU(a * b) = -RD(-a * b)
+__device__ __forceinline__ double ru_mul(double a, double b) { return -__builtin_fma(-a, b, 0.0); }
 // a - b*c, rounded down; exact whenever the true value is representable.
-__device__ __forceinline__ double fms_rd(double a, double b, double c) {
-    double r;
-    __asm__ volatile("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(b), "v"(c), "v"(a));
-    return r;
-}
+__device__ __forceinline__ double fms_rd(double a, double b, double c) { return __builtin_fma(-b, c, a); }
 // RU(a*b + c) = -RD(-a*b - c)
-__device__ __forceinline__ double fma_ru(double a, double b, double c) {
-    double r;
-    __asm__ volatile("v_fma_f64 %0, -%1, %2, -%3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return -r;
-}
+__device__ __forceinline__ double fma_ru(double a, double b, double c) { return -__builtin_fma(-a, b, -c); }
 
 __device__ __forceinline__ std::uint64_t bits(double x) { return __builtin_bit_cast(std::uint64_t, x); }
 __device__ __forceinline__ double from_bits(std::uint64_t u) { return __builtin_bit_cast(double, u); }

@@ -58,26 +58,60 @@ template<typename F> int guarded(F&& f) {
     catch(const std::exception& e) { t_error = e.what(); return SQD_ERR_INTERNAL; }
 }
 
+// Device-resident control block of a core: the counters, every list counter
+// and the round's pop range, in one allocation so that one copy brings the
+// whole state to the host.
+struct DeviceState {
+    dev::DeviceCounters counters;
+    unsigned long long stack_count, fin_count, spl_count, crit_count, scratch_count;
+    unsigned long long round_k, round_base;   // the round pops [round_base, round_base + round_k)
+    unsigned long long host_n;                // input count of host-driven calls
+    unsigned long long frontier_full;         // nonzero: a round could not pop anything
+    unsigned long long level_count[64];       // output counter of (mode, level): mode * 16 + level
+};
+
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long n, unsigned long long v) {
-    for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-        i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) p[i] = v;
-}
-// Entry 0 of a box list (SoA, stride `stride`) from a [lb..., ub...] box; count = 1.
-__global__ void k_box_to_list(const double* box, int n_args, double* lb, double* ub, unsigned* depth,
-                              unsigned long long stride, unsigned long long* count) {
-    const int a = static_cast<int>(threadIdx.x);
+// Start of a proof (or of a host-driven call): counters, stack and critical
+// counts, and every level counter to zero.
+__global__ void k_state_reset(DeviceState* st) {
+    const unsigned t = threadIdx.x;
     if(blockIdx.x != 0) return;
-    if(a < n_args) {
-        lb[static_cast<unsigned long long>(a) * stride] = box[a];
-        ub[static_cast<unsigned long long>(a) * stride] = box[n_args + a];
+    if(t == 0) {
+        st->counters = dev::DeviceCounters{0, 0, 0, 0};
+        st->stack_count = st->fin_count = st->spl_count = st->crit_count = st->scratch_count = 0;
+        st->round_k = st->round_base = st->host_n = st->frontier_full = 0;
     }
-    if(a == 0) {
-        if(depth) depth[0] = 0;
-        *count = 1;
+    if(t < 64) st->level_count[t] = 0;
+}
+// Start of a round: pop K = min(k_max, S, room / (fanout - 1)) cuboids from
+// the top of the stack (their slots are reused by the children the round
+// pushes), and clear the round's list counters.
+__global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigned long long stack_cap,
+                              unsigned long long fanout) {
+    const unsigned t = threadIdx.x;
+    if(blockIdx.x != 0) return;
+    if(t == 0) {
+        const unsigned long long S = st->stack_count;
+        unsigned long long K = S < k_max ? S : k_max;
+        if(fanout > 1) {
+            const unsigned long long room = stack_cap > S ? stack_cap - S : 0;
+            const unsigned long long lim = room / (fanout - 1);
+            K = K < lim ? K : lim;
+        }
+        if(K == 0 && S > 0) st->frontier_full = 1;
+        st->round_k = K;
+        st->round_base = S - K;
+        st->stack_count = S - K;
+        st->fin_count = 0;
+        st->spl_count = 0;
     }
+    if(t < 64) st->level_count[t] = 0;
+}
+// Level counters to zero (between chunks of a host-driven call).
+__global__ void k_levels_reset(DeviceState* st) {
+    if(blockIdx.x == 0 && threadIdx.x < 64) st->level_count[threadIdx.x] = 0;
 }
 // hull record initial values: min-key, max-key, zero-sign lb, zero-sign ub
 __global__ void k_init_hull(unsigned long long* p, unsigned long long n_records) {
@@ -97,7 +131,7 @@ template<typename T> T* dalloc(std::size_t n) {
     return static_cast<T*>(p);
 }
 
-// A device box list with its own storage.
+// A device box list with its own storage; its counter lives in the DeviceState.
 struct DevList {
     int N = 0;
     unsigned long long cap = 0;
@@ -107,25 +141,26 @@ struct DevList {
     unsigned* origin = nullptr;
     unsigned* leaf = nullptr;
     unsigned long long* count = nullptr;
-    void create(int n_args, unsigned long long capacity, bool with_depth, bool with_origin) {
+    void create(int n_args, unsigned long long capacity, bool with_depth, bool with_origin, unsigned long long* counter) {
         N = n_args;
         cap = capacity;
         lb = dalloc<double>(static_cast<std::size_t>(cap) * N);
         ub = dalloc<double>(static_cast<std::size_t>(cap) * N);
         if(with_depth) depth = dalloc<unsigned>(cap);
         if(with_origin) { origin = dalloc<unsigned>(cap); leaf = dalloc<unsigned>(cap); }
-        count = dalloc<unsigned long long>(1);
+        count = counter;
     }
     void destroy() {
         for(void* p : {static_cast<void*>(lb), static_cast<void*>(ub), static_cast<void*>(depth),
-                       static_cast<void*>(origin), static_cast<void*>(leaf), static_cast<void*>(count)})
+                       static_cast<void*>(origin), static_cast<void*>(leaf)})
             if(p) (void)hipFree(p);
         lb = ub = nullptr; depth = origin = leaf = nullptr; count = nullptr;
     }
-    dev::BoxList view() const {
+    dev::BoxList view() const { return view(count); }
+    dev::BoxList view(unsigned long long* counter) const {
         dev::BoxList v;
         v.lb = lb; v.ub = ub; v.depth = depth; v.origin = origin; v.leaf = leaf;
-        v.stride = cap; v.cap = cap; v.count = count;
+        v.stride = cap; v.cap = cap; v.count = counter;
         return v;
     }
 };
@@ -157,8 +192,10 @@ struct sqd_core {
     double* d_consts = nullptr;
     double* d_box_ct = nullptr;     // 2N: outward compile-time box
     double* d_box_init = nullptr;   // 2N: initial runtime box
-    sqd::dev::DeviceCounters* d_counters = nullptr;
+    sqd::DeviceState* d_state = nullptr;
+    sqd::DeviceState* h_state = nullptr;   // pinned host copy
     unsigned long long* d_hull = nullptr;
+    bool hull_dirty = true;                // hull records need a full reset before use
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
     unsigned long long k_max = 0;    // dynamic cuboids per round
@@ -247,102 +284,97 @@ void set_count(sqd_core* c, unsigned long long* p, unsigned long long v) {
         return hipGetLastError();
     });
 }
-unsigned long long get_count(sqd_core* c, unsigned long long* p) {
-    unsigned long long v = 0;
-    hip_check(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync count");
-    sync(c);
-    return v;
+void reset_state(sqd_core* c) {
+    launch(c, KIND_SET_COUNT, "k_state_reset", [&] {
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, c->stream, c->d_state);
+        return hipGetLastError();
+    });
 }
-void reset_counters(sqd_core* c) {
-    hip_check(hipMemsetAsync(c->d_counters, 0, sizeof(dev::DeviceCounters), c->stream), "hipMemsetAsync counters");
-}
-// Reads the counters (and, if given, one more device count in the same sync).
-sqd_counts read_counters(sqd_core* c, const unsigned long long* extra = nullptr, unsigned long long* extra_out = nullptr) {
-    dev::DeviceCounters h{};
-    hip_check(hipMemcpyAsync(&h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream), "copy counters");
-    if(extra) hip_check(hipMemcpyAsync(extra_out, extra, sizeof *extra_out, hipMemcpyDeviceToHost, c->stream), "copy count");
+// Brings the whole device state to the pinned host copy (one copy, one sync)
+// and checks the overflow flags.
+const DeviceState& read_state(sqd_core* c) {
+    hip_check(hipMemcpyAsync(c->h_state, c->d_state, sizeof(DeviceState), hipMemcpyDeviceToHost, c->stream),
+              "copy state");
     sync(c);
-    if(h.overflow) throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
+    if(c->h_state->counters.overflow)
+        throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
+    if(c->h_state->frontier_full)
+        throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
+    return *c->h_state;
+}
+sqd_counts counts_of(const DeviceState& h) {
     sqd_counts r;
-    r.num_cuboids = h.cuboids;
-    r.num_leaf_cuboids = h.leaves;
-    r.num_critical_cuboids = h.criticals;
+    r.num_cuboids = h.counters.cuboids;
+    r.num_leaf_cuboids = h.counters.leaves;
+    r.num_critical_cuboids = h.counters.criticals;
     r.num_repeated_nodes = 0;
     return r;
 }
-void add_counts(sqd_counts* out, const sqd_counts& c) {
-    if(!out) return;
-    out->num_cuboids += c.num_cuboids;
-    out->num_leaf_cuboids += c.num_leaf_cuboids;
-    out->num_critical_cuboids += c.num_critical_cuboids;
-    out->num_repeated_nodes += c.num_repeated_nodes;
-}
 
-dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, unsigned long long base, unsigned long long fixed,
-                          const dev::BoxList& out) {
+dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned long long* base,
+                          const unsigned long long* count, const dev::BoxList& out) {
     dev::LevelArgs a;
     a.in = in;
     a.out = out;
     a.in_base = base;
-    a.in_fixed_count = fixed;
+    a.in_count = count;
     a.hull = c->d_hull;
-    a.counters = c->d_counters;
+    a.counters = &c->d_state->counters;
     a.consts = c->d_consts;
     a.generations = c->settings.generation_count;
     return a;
 }
 
-// Static levels over the inputs taken from `in`: `n` of them when n > 0, else
-// the list's device counter, which is at most `max_items`.  NONFINAL joins the
-// leaves into d_hull; FINAL appends critical leaves to `crit_out`.
-void run_static(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long base, unsigned long long n,
-                unsigned long long max_items, const dev::BoxList& crit_out) {
+// Static levels over the inputs [*base, *base + *count) of `in` (base may be
+// null: 0), at most `max_items` of them.  NONFINAL joins the leaves into d_hull;
+// FINAL appends critical leaves to `crit_out`.  Level L writes lvl[L % 2] with
+// the counter level_count[mode * 16 + L], which must be zero on entry.
+void run_static(sqd_core* c, int mode, const dev::BoxList& in, const unsigned long long* base,
+                const unsigned long long* count, unsigned long long max_items, const dev::BoxList& crit_out) {
     const int S = c->ks->NSTAT;
     dev::BoxList prev = in;
-    unsigned long long prev_base = base, prev_fixed = n;
-    unsigned long long items = n ? n : max_items;
+    const unsigned long long* prev_base = base;
+    const unsigned long long* prev_count = count;
+    unsigned long long items = max_items;
     for(int L = 0; L < S; ++L) {
         const bool last = (L == S - 1);
-        dev::BoxList out = last ? crit_out : c->lvl[L % 2].view();
-        if(!last) set_count(c, out.count, 0);
+        dev::BoxList out = last ? crit_out : c->lvl[L % 2].view(&c->d_state->level_count[mode * 16 + L]);
         dev::BoxList src = prev;
         if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
-        dev::LevelArgs a = level_args(c, src, prev_base, prev_fixed, out);
+        dev::LevelArgs a = level_args(c, src, prev_base, prev_count, out);
         const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
         launch(c, mode * 16 + L, "static level launch",
                [&] { return c->ks->static_level(mode, L, a, children, c->cus, c->grid_override, c->stream); });
-        prev = c->lvl[L % 2].view();
-        prev_base = 0;
-        prev_fixed = 0;
+        prev = out;
+        prev_base = nullptr;
+        prev_count = out.count;
         items = std::min<unsigned long long>(children, c->lvl[L % 2].cap);
     }
 }
 
-// Dynamic levels (SPLIT or INIT) from `in` (n inputs when n > 0, else the
-// list's counter, at most `max_items`) to `out` (appending to its counter).
-void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, unsigned long long n, unsigned long long max_items,
-                 const dev::BoxList& out) {
+// Dynamic levels (SPLIT or INIT) from the inputs [0, *count) of `in` (at most
+// `max_items`) to `out`, appending to its counter.
+void run_dynamic(sqd_core* c, int mode, const dev::BoxList& in, const unsigned long long* count,
+                 unsigned long long max_items, const dev::BoxList& out) {
     const int D = c->ks->NDYN;
     dev::BoxList prev = in;
-    unsigned long long prev_fixed = n;
-    unsigned long long items = n ? n : max_items;
+    const unsigned long long* prev_count = count;
+    unsigned long long items = max_items;
     for(int J = 0; J < D; ++J) {
         const bool last = (J == D - 1);
-        dev::BoxList o = last ? out : c->lvl[J % 2].view();
-        if(!last) set_count(c, o.count, 0);
+        dev::BoxList o = last ? out : c->lvl[J % 2].view(&c->d_state->level_count[mode * 16 + J]);
         o.origin = nullptr;
         o.leaf = nullptr;
-        if(!last) o.depth = c->lvl[J % 2].depth;
         dev::BoxList src = prev;
         src.origin = nullptr;
         src.leaf = nullptr;
-        dev::LevelArgs a = level_args(c, src, 0, prev_fixed, o);
+        dev::LevelArgs a = level_args(c, src, nullptr, prev_count, o);
         const unsigned long long children =
             items * (mode == dev::INIT ? c->dyn_initial[static_cast<std::size_t>(J)] : c->dyn_subs[static_cast<std::size_t>(J)]);
         launch(c, mode * 16 + J, "dynamic level launch",
                [&] { return c->ks->dynamic_level(mode, J, a, children, c->cus, c->grid_override, c->stream); });
-        prev = c->lvl[J % 2].view();
-        prev_fixed = 0;
+        prev = o;
+        prev_count = o.count;
         items = std::min<unsigned long long>(children, c->lvl[J % 2].cap);
     }
 }
@@ -373,58 +405,63 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
     sync(c);
 }
 
-void launch_initial_box(sqd_core* c) {
-    launch(c, KIND_INITIAL_BOX, "initial box", [&] {
-        return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, c->stream);
-    });
-}
-
-// Initial queue onto the (empty) stack, all on the device; returns its length.
-unsigned long long build_initial_queue(sqd_core* c) {
-    launch_initial_box(c);
-    launch(c, KIND_SET_COUNT, "k_box_to_list", [&] {
-        hipLaunchKernelGGL(k_box_to_list, dim3(1), dim3(64), 0, c->stream, c->d_box_init, c->N, c->scratch.lb,
-                           c->scratch.ub, c->scratch.depth, c->scratch.cap, c->scratch.count);
-        return hipGetLastError();
-    });
-    set_count(c, c->stack.count, 0);
-    run_dynamic(c, dev::INIT, c->scratch.view(), 1, 1, c->stack.view());
-    return get_count(c, c->stack.count);
-}
-
-void init_hull(sqd_core* c, unsigned long long K) {
-    launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
-        hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, K * static_cast<unsigned long long>(c->N));
-        return hipGetLastError();
-    });
-}
-
-// One round over the top K cuboids of the stack; returns the new stack size.
-unsigned long long run_round(sqd_core* c, unsigned long long S, unsigned long long K) {
-    const unsigned long long base = S - K;
-    const int N = c->N;
-    dev::BoxList items = c->stack.view();
-    if(c->ks->NSTAT > 0) {
-        init_hull(c, K);
-        run_static(c, dev::NONFINAL, items, base, K, K, c->crit.view());
+// Hull records are cleared by the kernels that consume them; after a failed
+// call they are reset in full before the next use.
+void prepare_hull(sqd_core* c) {
+    if(c->hull_dirty && c->ks->NSTAT > 0) {
+        launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
+            hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, c->k_max * static_cast<unsigned long long>(c->N));
+            return hipGetLastError();
+        });
     }
-    set_count(c, c->fin.count, 0);
-    set_count(c, c->spl.count, 0);
+    c->hull_dirty = true;   // until the call completes
+}
+void levels_reset(sqd_core* c) {
+    launch(c, KIND_SET_COUNT, "k_levels_reset", [&] {
+        hipLaunchKernelGGL(k_levels_reset, dim3(1), dim3(64), 0, c->stream, c->d_state);
+        return hipGetLastError();
+    });
+}
+
+// Initial runtime box into d_box_init and, if `to_scratch`, as the single
+// entry of the scratch list.
+void launch_initial_box(sqd_core* c, bool to_scratch) {
+    dev::BoxList list{};
+    if(to_scratch) list = c->scratch.view();
+    launch(c, KIND_INITIAL_BOX, "initial box", [&] {
+        return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, list, c->stream);
+    });
+}
+
+// Enqueues the initial queue onto the (empty) stack; nothing is waited for.
+void enqueue_initial_queue(sqd_core* c) {
+    launch_initial_box(c, true);
+    run_dynamic(c, dev::INIT, c->scratch.view(), c->scratch.count, 1, c->stack.view());
+}
+
+// Enqueues one round over the top of the stack: pop (on the device), non-final
+// static levels with hull join, hull sorting into final and split lists, final
+// pass, dynamic split pushing the children.  Nothing is waited for.
+void enqueue_round(sqd_core* c) {
+    DeviceState* st = c->d_state;
+    launch(c, KIND_SET_COUNT, "k_round_begin", [&] {
+        hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, c->stream, st, c->k_max, c->stack.cap,
+                           c->split_fanout);
+        return hipGetLastError();
+    });
+    const unsigned long long K = c->k_max;   // bound on the round's size
+    dev::BoxList items = c->stack.view();
+    if(c->ks->NSTAT > 0) run_static(c, dev::NONFINAL, items, &st->round_base, &st->round_k, K, c->crit.view());
     launch(c, KIND_HULL_FINALIZE, "hull finalize", [&] {
-        return c->ks->hull_finalize(items, base, K, c->d_hull, c->fin.view(), c->spl.view(), c->d_counters,
-                                    c->settings.generation_count, c->stream);
+        return c->ks->hull_finalize(items, &st->round_base, &st->round_k, K, c->d_hull, c->fin.view(), c->spl.view(),
+                                    &st->counters, c->settings.generation_count, c->stream);
     });
-    set_count(c, c->stack.count, base);
-    // final pass over hulls at the last generation
-    if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, K, c->crit.view());
+    if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, K, c->crit.view());
     else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
-        return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters, static_cast<unsigned>((K + 255) / 256),
-                                     c->stream);
+        return c->ks->final_nostatic(c->fin.view(), c->crit.view(), &st->counters,
+                                     static_cast<unsigned>((K + 255) / 256), c->stream);
     });
-    // dynamic split of the other hulls, children pushed onto the stack
-    dev::BoxList spl = c->spl.view();
-    run_dynamic(c, dev::SPLIT, spl, 0, K, c->stack.view());
-    return get_count(c, c->stack.count);
+    run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, K, c->stack.view());
 }
 
 } // namespace
@@ -540,15 +577,20 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         kmax = std::min(kmax, lvl_cap / worst_split);
         if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
         c->k_max = kmax;
-        c->stack.create(c->N, std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1), true, false);
-        for(auto& l : c->lvl) l.create(c->N, lvl_cap, true, true);
-        c->fin.create(c->N, kmax, true, false);
-        c->spl.create(c->N, kmax, true, false);
-        c->crit.create(c->N, s.critical_capacity, false, false);
-        c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false);
+        c->d_state = dalloc<DeviceState>(1);
+        void* hs = nullptr;
+        hip_check(hipHostMalloc(&hs, sizeof(DeviceState), hipHostMallocDefault), "hipHostMalloc");
+        c->h_state = static_cast<DeviceState*>(hs);
+        DeviceState* st = c->d_state;
+        c->stack.create(c->N, std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1), true, false,
+                        &st->stack_count);
+        for(auto& l : c->lvl) l.create(c->N, lvl_cap, true, true, nullptr);
+        c->fin.create(c->N, kmax, true, false, &st->fin_count);
+        c->spl.create(c->N, kmax, true, false, &st->spl_count);
+        c->crit.create(c->N, s.critical_capacity, false, false, &st->crit_count);
+        c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false, &st->scratch_count);
         c->d_hull = dalloc<unsigned long long>(static_cast<std::size_t>(kmax) * c->N * 4);
         c->d_has = dalloc<unsigned char>(static_cast<std::size_t>(kmax));
-        c->d_counters = dalloc<dev::DeviceCounters>(1);
         c->d_consts = dalloc<double>(2 * static_cast<std::size_t>(std::max(1, c->ks->NC)));
         hip_check(hipMemcpy(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
                             hipMemcpyHostToDevice), "upload constants");
@@ -568,9 +610,10 @@ void sqd_core_destroy(sqd_core* c) {
     c->stack.destroy();
     for(auto& l : c->lvl) l.destroy();
     c->fin.destroy(); c->spl.destroy(); c->crit.destroy(); c->scratch.destroy();
-    for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_counters),
+    for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_state),
                    static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
         if(p) (void)hipFree(p);
+    if(c->h_state) (void)hipHostFree(c->h_state);
     for(hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -580,7 +623,7 @@ int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
     return guarded([&] {
         if(!c || !lb || !ub) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        launch_initial_box(c);
+        launch_initial_box(c, false);
         sync(c);
         hip_check(hipMemcpy(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
         hip_check(hipMemcpy(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
@@ -591,11 +634,14 @@ int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        reset_counters(c);
-        unsigned long long len = build_initial_queue(c);
+        reset_state(c);
+        enqueue_initial_queue(c);
+        const DeviceState& h = read_state(c);
+        const unsigned long long len = h.stack_count;
+        const sqd_counts r = counts_of(h);
         *n = len;
         if(lb && ub) download_soa(c, c->stack, std::min<unsigned long long>(len, capacity), lb, ub, capacity);
-        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+        if(counts) *counts = r;
     });
 }
 
@@ -604,7 +650,8 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
     return guarded([&] {
         if(!c || (n && (!lb || !ub || !hull_lb || !hull_ub || !has_hull))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        reset_counters(c);
+        reset_state(c);
+        prepare_hull(c);
         std::vector<double> clb, cub, hlb, hub;
         for(std::size_t off = 0; off < n; off += c->k_max) {
             const std::size_t m = std::min<std::size_t>(c->k_max, n - off);
@@ -616,10 +663,8 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
                     cub[static_cast<std::size_t>(a) * m + i] = ub[static_cast<std::size_t>(a) * n + off + i];
                 }
             upload_soa(c, c->stack, clb.data(), cub.data(), m, 0);
-            if(c->ks->NSTAT > 0) {
-                init_hull(c, m);
-                run_static(c, dev::NONFINAL, c->stack.view(), 0, m, m, c->crit.view());
-            }
+            levels_reset(c);
+            if(c->ks->NSTAT > 0) run_static(c, dev::NONFINAL, c->stack.view(), nullptr, c->stack.count, m, c->crit.view());
             launch(c, KIND_HULL_DECODE, "decode", [&] {
                 return c->ks->hull_decode(c->stack.view(), m, c->d_hull, c->scratch.view(), c->d_has, c->stream);
             });
@@ -633,7 +678,8 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
                     hull_ub[static_cast<std::size_t>(a) * n + off + i] = hub[static_cast<std::size_t>(a) * m + i];
                 }
         }
-        sqd_counts r = read_counters(c);
+        sqd_counts r = counts_of(read_state(c));
+        c->hull_dirty = false;
         if(c->ks->NSTAT == 0) r.num_leaf_cuboids += n;  // without static variables each box is its own leaf
         if(counts) *counts = r;
     });
@@ -644,19 +690,20 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
     return guarded([&] {
         if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        reset_counters(c);
-        set_count(c, c->crit.count, 0);
         if(n > c->fin.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one final pass");
+        reset_state(c);
         upload_soa(c, c->fin, lb, ub, n, 0);
-        if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), 0, 0, n, c->crit.view());
+        if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, n, c->crit.view());
         else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
-            return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_counters,
+            return c->ks->final_nostatic(c->fin.view(), c->crit.view(), &c->d_state->counters,
                                          static_cast<unsigned>((n + 255) / 256 + 1), c->stream);
         });
-        unsigned long long k = get_count(c, c->crit.count);
+        const DeviceState& h = read_state(c);
+        const unsigned long long k = h.crit_count;
+        const sqd_counts r = counts_of(h);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
-        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+        if(counts) *counts = r;
     });
 }
 
@@ -665,15 +712,16 @@ int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
     return guarded([&] {
         if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        reset_counters(c);
         if(n > c->spl.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one split");
+        reset_state(c);
         upload_soa(c, c->spl, lb, ub, n, 1);
-        set_count(c, c->stack.count, 0);
-        run_dynamic(c, dev::SPLIT, c->spl.view(), 0, n, c->stack.view());
-        unsigned long long k = get_count(c, c->stack.count);
+        run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, n, c->stack.view());
+        const DeviceState& h = read_state(c);
+        const unsigned long long k = h.stack_count;
+        const sqd_counts r = counts_of(h);
         if(n_out) *n_out = k;
         if(out_lb && out_ub) download_soa(c, c->stack, std::min<unsigned long long>(k, capacity), out_lb, out_ub, capacity);
-        if(counts) { *counts = sqd_counts{}; add_counts(counts, read_counters(c)); }
+        if(counts) *counts = r;
     });
 }
 
@@ -683,19 +731,19 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
         if(!c) throw std::invalid_argument("null core");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto t0 = std::chrono::steady_clock::now();
-        reset_counters(c);
-        set_count(c, c->crit.count, 0);
-        unsigned long long S = build_initial_queue(c);
-        const unsigned long long cap = c->stack.cap;
-        while(S > 0) {
-            unsigned long long room = cap > S ? cap - S : 0;
-            unsigned long long K = std::min(c->k_max, S);
-            if(c->split_fanout > 1) K = std::min(K, room / (c->split_fanout - 1));
-            if(K == 0) throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
-            S = run_round(c, S, K);
-        }
-        unsigned long long k = 0;
-        sqd_counts r = read_counters(c, c->crit.count, &k);
+        reset_state(c);
+        prepare_hull(c);
+        enqueue_initial_queue(c);
+        // One host round trip per round: the round's size is decided on the
+        // device, the host only learns whether the frontier is empty.
+        const DeviceState* h = nullptr;
+        do {
+            enqueue_round(c);
+            h = &read_state(c);
+        } while(h->stack_count > 0);
+        c->hull_dirty = false;
+        const sqd_counts r = counts_of(*h);
+        const unsigned long long k = h->crit_count;
         auto t1 = std::chrono::steady_clock::now();
         if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
         if(counts) *counts = r;

@@ -8,6 +8,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <map>
+#include <utility>
+#include <vector>
 #include <cstdio>
 
 namespace ivarp {
@@ -52,65 +54,119 @@ struct Emitter {
     }
     std::string v(int id) { return "v" + std::to_string(id); }
     std::string b(int id) { return "p" + std::to_string(id); }
-    // Straight-line evaluation of the sub-DAG below `root`.
+    // Evaluation of the sub-DAG below `root`, lazy like the reference's
+    // evaluation: an if_then_else evaluates its then-branch only where the
+    // condition is possibly true and its else-branch only where it is possibly
+    // false (math_fn/eval/if_then_else.hpp:64-82), and || / && stop at the first
+    // definitely true / not possibly true operand (predicate_eval/n_ary.hpp:63-135).
+    // Every node is computed once, in the innermost scope that contains all of
+    // its uses: the common prefix of the (conditional node, operand) paths from
+    // the root to it.  Results are those of the full evaluation, since all nodes
+    // are pure.
+    using Ctx = std::vector<std::pair<int, int>>;
     std::string eval(int root, const std::string& indent) {
         std::vector<int> order;
         std::vector<char> seen(p.nodes.size(), 0);
         collect(root, order, seen);
-        std::ostringstream o;
-        for(int id : order) {
+        std::map<int, Ctx> ctx;
+        ctx[root] = {};
+        for(auto it = order.rbegin(); it != order.rend(); ++it) {
+            const int id = *it;
             const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
-            auto K = [&](std::size_t i) { return n.kids[i]; };
-            auto KN = [&](std::size_t i) -> const flat::FNode& { return p.nodes[static_cast<std::size_t>(n.kids[i])]; };
-            o << indent;
-            switch(n.op) {
-                case FOp::Const: {
-                    int c = cidx(id);
-                    o << "const IV " << v(id) << "{C[" << 2 * c << "], C[" << 2 * c + 1 << "], " << (n.c_undef ? "true" : "false") << "};";
-                    break;
-                }
-                case FOp::Arg: o << "const IV " << v(id) << "{lb[" << n.arg << "], ub[" << n.arg << "], false};"; break;
-                case FOp::Neg: o << "const IV " << v(id) << " = iv_neg(" << v(K(0)) << ");"; break;
-                case FOp::Add: o << "const IV " << v(id) << " = iv_add(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Sub: o << "const IV " << v(id) << " = iv_sub(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Mul:
-                    o << "const IV " << v(id) << " = iv_mul<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
-                      << tri_nn(KN(1)) << ", " << tri_np(KN(1)) << ", "
-                      << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
-                    break;
-                case FOp::Div:
-                    o << "const IV " << v(id) << " = iv_div<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
-                      << tri_pos(KN(1)) << ", " << tri_neg(KN(1)) << ", "
-                      << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
-                    break;
-                case FOp::Sqrt: o << "const IV " << v(id) << " = iv_sqrt<" << tri_nn(KN(0)) << ">(" << v(K(0)) << ");"; break;
-                case FOp::Square: o << "const IV " << v(id) << " = iv_square(" << v(K(0)) << ");"; break;
-                case FOp::Max: case FOp::Min: {
-                    const char* f = n.op == FOp::Max ? "iv_max" : "iv_min";
-                    std::string acc = v(K(n.kids.size() - 1));
-                    for(std::size_t i = n.kids.size() - 1; i-- > 0;) acc = std::string(f) + "(" + v(K(i)) + ", " + acc + ")";
-                    o << "const IV " << v(id) << " = " << acc << ";";
-                    break;
-                }
-                case FOp::Ite: o << "const IV " << v(id) << " = iv_ite(" << b(K(0)) << ", " << v(K(1)) << ", " << v(K(2)) << ");"; break;
-                case FOp::Acos: throw std::runtime_error("acos is not supported by the GPU code generator yet");
-                case FOp::BoolConst: o << "const IB " << b(id) << "{" << (n.b_def ? "true" : "false") << ", " << (n.b_poss ? "true" : "false") << "};"; break;
-                case FOp::Lt: o << "const IB " << b(id) << " = ib_lt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Le: o << "const IB " << b(id) << " = ib_le(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Gt: o << "const IB " << b(id) << " = ib_gt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Ge: o << "const IB " << b(id) << " = ib_ge(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Eq: o << "const IB " << b(id) << " = ib_eq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Neq: o << "const IB " << b(id) << " = ib_neq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
-                case FOp::Not: o << "const IB " << b(id) << " = ib_not(" << b(K(0)) << ");"; break;
-                case FOp::Or: case FOp::And: {
-                    const char* f = n.op == FOp::Or ? "ib_or" : "ib_and";
-                    std::string acc = b(K(0));
-                    for(std::size_t i = 1; i < n.kids.size(); ++i) acc = std::string(f) + "(" + acc + ", " + b(K(i)) + ")";
-                    o << "const IB " << b(id) << " = " << acc << ";";
-                    break;
-                }
+            const Ctx& c = ctx.at(id);
+            const bool lazy = n.op == FOp::Ite || n.op == FOp::Or || n.op == FOp::And;
+            for(std::size_t i = 0; i < n.kids.size(); ++i) {
+                Ctx ck = c;
+                if(lazy && i > 0) ck.push_back({id, static_cast<int>(i)});
+                const int k = n.kids[i];
+                auto f = ctx.find(k);
+                if(f == ctx.end()) { ctx[k] = ck; continue; }
+                Ctx& cur = f->second;
+                std::size_t m = 0;
+                while(m < cur.size() && m < ck.size() && cur[m] == ck[m]) ++m;
+                cur.resize(m);
             }
-            o << "\n";
+        }
+        std::ostringstream o;
+        emit_scope(order, ctx, {}, indent, o);
+        return o.str();
+    }
+    void emit_scope(const std::vector<int>& order, const std::map<int, Ctx>& ctx, const Ctx& scope,
+                    const std::string& indent, std::ostringstream& o) {
+        for(int id : order) {
+            if(ctx.at(id) != scope) continue;
+            const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
+            auto sub = [&](int i) { Ctx c = scope; c.push_back({id, i}); return c; };
+            if(n.op == FOp::Ite) {
+                const std::string c = b(n.kids[0]), t = "t" + std::to_string(id), e = "e" + std::to_string(id);
+                o << indent << "IV " << t << "{0.0, 0.0, false}, " << e << "{0.0, 0.0, false};\n";
+                o << indent << "if(" << c << ".poss) {\n";
+                emit_scope(order, ctx, sub(1), indent + "    ", o);
+                o << indent << "    " << t << " = " << v(n.kids[1]) << ";\n" << indent << "}\n";
+                o << indent << "if(!" << c << ".def) {\n";
+                emit_scope(order, ctx, sub(2), indent + "    ", o);
+                o << indent << "    " << e << " = " << v(n.kids[2]) << ";\n" << indent << "}\n";
+                o << indent << "const IV " << v(id) << " = iv_ite(" << c << ", " << t << ", " << e << ");\n";
+                continue;
+            }
+            if(n.op == FOp::Or || n.op == FOp::And) {
+                const bool is_or = n.op == FOp::Or;
+                o << indent << "IB " << b(id) << " = " << b(n.kids[0]) << ";\n";
+                for(std::size_t i = 1; i < n.kids.size(); ++i) {
+                    o << indent << "if(" << (is_or ? "!" : "") << b(id) << (is_or ? ".def" : ".poss") << ") {\n";
+                    emit_scope(order, ctx, sub(static_cast<int>(i)), indent + "    ", o);
+                    o << indent << "    " << b(id) << " = " << (is_or ? "ib_or(" : "ib_and(") << b(id) << ", "
+                      << b(n.kids[i]) << ");\n" << indent << "}\n";
+                }
+                continue;
+            }
+            o << indent << node_code(id) << "\n";
+        }
+    }
+    std::string node_code(int id) {
+        const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
+        auto K = [&](std::size_t i) { return n.kids[i]; };
+        auto KN = [&](std::size_t i) -> const flat::FNode& { return p.nodes[static_cast<std::size_t>(n.kids[i])]; };
+        std::ostringstream o;
+        switch(n.op) {
+            case FOp::Const: {
+                int c = cidx(id);
+                o << "const IV " << v(id) << "{C[" << 2 * c << "], C[" << 2 * c + 1 << "], " << (n.c_undef ? "true" : "false") << "};";
+                break;
+            }
+            case FOp::Arg: o << "const IV " << v(id) << "{lb[" << n.arg << "], ub[" << n.arg << "], false};"; break;
+            case FOp::Neg: o << "const IV " << v(id) << " = iv_neg(" << v(K(0)) << ");"; break;
+            case FOp::Add: o << "const IV " << v(id) << " = iv_add(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Sub: o << "const IV " << v(id) << " = iv_sub(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Mul:
+                o << "const IV " << v(id) << " = iv_mul<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
+                  << tri_nn(KN(1)) << ", " << tri_np(KN(1)) << ", "
+                  << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
+                break;
+            case FOp::Div:
+                o << "const IV " << v(id) << " = iv_div<" << tri_nn(KN(0)) << ", " << tri_np(KN(0)) << ", "
+                  << tri_pos(KN(1)) << ", " << tri_neg(KN(1)) << ", "
+                  << ((ct_finite(KN(0)) && ct_finite(KN(1))) ? "true" : "false") << ">(" << v(K(0)) << ", " << v(K(1)) << ");";
+                break;
+            case FOp::Sqrt: o << "const IV " << v(id) << " = iv_sqrt<" << tri_nn(KN(0)) << ">(" << v(K(0)) << ");"; break;
+            case FOp::Square: o << "const IV " << v(id) << " = iv_square(" << v(K(0)) << ");"; break;
+            case FOp::Max: case FOp::Min: {
+                const char* f = n.op == FOp::Max ? "iv_max" : "iv_min";
+                std::string acc = v(K(n.kids.size() - 1));
+                for(std::size_t i = n.kids.size() - 1; i-- > 0;) acc = std::string(f) + "(" + v(K(i)) + ", " + acc + ")";
+                o << "const IV " << v(id) << " = " << acc << ";";
+                break;
+            }
+            case FOp::Ite: case FOp::Or: case FOp::And: throw std::logic_error("lazy node emitted as plain code");
+            case FOp::Acos: throw std::runtime_error("acos is not supported by the GPU code generator yet");
+            case FOp::BoolConst: o << "const IB " << b(id) << "{" << (n.b_def ? "true" : "false") << ", " << (n.b_poss ? "true" : "false") << "};"; break;
+            case FOp::Lt: o << "const IB " << b(id) << " = ib_lt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Le: o << "const IB " << b(id) << " = ib_le(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Gt: o << "const IB " << b(id) << " = ib_gt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Ge: o << "const IB " << b(id) << " = ib_ge(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Eq: o << "const IB " << b(id) << " = ib_eq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Neq: o << "const IB " << b(id) << " = ib_neq(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
+            case FOp::Not: o << "const IB " << b(id) << " = ib_not(" << b(K(0)) << ");"; break;
         }
         return o.str();
     }
