@@ -122,6 +122,12 @@ int sqd_split(sqd_core* core, const double* lb, const double* ub, size_t n,
 int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_ub,
               size_t capacity, size_t* n_critical, double* seconds);
 
+/* The critical cuboids found by the last sqd_prove on this core (they stay in
+ * device memory until the next call): up to `capacity` boxes (SoA, stride
+ * capacity); *n receives the total found.  The reference hands each critical
+ * to its on_critical callback (prover/proof_driver.hpp:499-509). */
+int sqd_core_criticals(sqd_core* core, double* crit_lb, double* crit_ub, size_t capacity, size_t* n);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Device time per kernel kind, from HIP events recorded on the core's stream

@@ -514,12 +514,15 @@ __global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList 
 }
 
 // Initial runtime box: outward compile-time box, then 50*N rounds of all bounds.
-// The box goes to `out` (N lb, N ub) and, when list.lb is set, to entry 0 of
-// `list` (depth 0) with the list's count set to 1.
+// First `zero_words` words at `zero` are cleared (a fresh control block).  The
+// box goes to `out` (N lb, N ub) and, when list.lb is set, to entry 0 of `list`
+// (depth 0) with the list's count set to 1.
 template<class P>
 __global__ void k_initial_box(const double* box_lb, const double* box_ub, const double* consts, double* out,
-                              BoxList list) {
+                              BoxList list, unsigned long long* zero, unsigned zero_words) {
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    for(unsigned i = threadIdx.x; zero && i < zero_words; i += blockDim.x) zero[i] = 0ull;
+    __threadfence_block();
     load_consts<P>(s_c, consts);
     enter_round_down();
     if(threadIdx.x != 0 || blockIdx.x != 0) return;

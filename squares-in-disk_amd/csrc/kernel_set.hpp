@@ -29,7 +29,7 @@ struct KernelSet {
                                 dev::BoxList spl, dev::DeviceCounters* c, unsigned gens, hipStream_t s);
     hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s);
     hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out,
-                              dev::BoxList list, hipStream_t s);
+                              dev::BoxList list, unsigned long long* zero, unsigned zero_words, hipStream_t s);
     hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
                               dev::BoxList out, unsigned char* has, hipStream_t s);
 };
@@ -118,8 +118,9 @@ hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, dev::DeviceCounter
 }
 template<class P>
 hipError_t initial_box(const double* lb, const double* ub, const double* consts, double* out, dev::BoxList list,
-                       hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out, list);
+                       unsigned long long* zero, unsigned zero_words, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out, list, zero,
+                       zero_words);
     return hipGetLastError();
 }
 

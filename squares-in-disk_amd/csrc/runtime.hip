@@ -70,6 +70,8 @@ struct DeviceState {
     unsigned long long level_count[64];       // output counter of (mode, level): mode * 16 + level
 };
 
+static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a block of 64-bit words");
+
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -424,18 +426,23 @@ void levels_reset(sqd_core* c) {
 }
 
 // Initial runtime box into d_box_init and, if `to_scratch`, as the single
-// entry of the scratch list.
-void launch_initial_box(sqd_core* c, bool to_scratch) {
+// entry of the scratch list.  With `reset`, the kernel first clears the whole
+// device state (as k_state_reset does).
+void launch_initial_box(sqd_core* c, bool to_scratch, bool reset) {
     dev::BoxList list{};
     if(to_scratch) list = c->scratch.view();
+    unsigned long long* zero = reset ? reinterpret_cast<unsigned long long*>(c->d_state) : nullptr;
+    const unsigned words = reset ? static_cast<unsigned>(sizeof(DeviceState) / sizeof(unsigned long long)) : 0u;
     launch(c, KIND_INITIAL_BOX, "initial box", [&] {
-        return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, list, c->stream);
+        return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, list, zero, words,
+                                  c->stream);
     });
 }
 
-// Enqueues the initial queue onto the (empty) stack; nothing is waited for.
+// Enqueues a cleared device state and the initial queue on the (empty) stack;
+// nothing is waited for.
 void enqueue_initial_queue(sqd_core* c) {
-    launch_initial_box(c, true);
+    launch_initial_box(c, true, true);
     run_dynamic(c, dev::INIT, c->scratch.view(), c->scratch.count, 1, c->stack.view());
 }
 
@@ -623,7 +630,7 @@ int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
     return guarded([&] {
         if(!c || !lb || !ub) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        launch_initial_box(c, false);
+        launch_initial_box(c, false, false);
         sync(c);
         hip_check(hipMemcpy(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
         hip_check(hipMemcpy(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
@@ -634,7 +641,6 @@ int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        reset_state(c);
         enqueue_initial_queue(c);
         const DeviceState& h = read_state(c);
         const unsigned long long len = h.stack_count;
@@ -731,7 +737,6 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
         if(!c) throw std::invalid_argument("null core");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto t0 = std::chrono::steady_clock::now();
-        reset_state(c);
         prepare_hull(c);
         enqueue_initial_queue(c);
         // One host round trip per round: the round's size is decided on the
@@ -749,6 +754,17 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
         if(counts) *counts = r;
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+    });
+}
+
+int sqd_core_criticals(sqd_core* c, double* crit_lb, double* crit_ub, size_t capacity, size_t* n) {
+    return guarded([&] {
+        if(!c || !n) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        const unsigned long long k = read_state(c).crit_count;
+        *n = k;
+        if(crit_lb && crit_ub && k)
+            download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
     });
 }
 

@@ -81,6 +81,7 @@ def lib():
             "sqd_split": (ctypes.c_int, [_P, _D, _D, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(Counts)]),
             "sqd_prove": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _D, _D, _SZ, ctypes.POINTER(_SZ),
                                          ctypes.POINTER(ctypes.c_double)]),
+            "sqd_core_criticals": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(_SZ)]),
             "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
@@ -100,7 +101,8 @@ def exported_symbols():
     return ["sqd_settings_default", "sqd_program_from_text", "sqd_program_from_lemma", "sqd_program_destroy",
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
-            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_set_profiling",
+            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_criticals",
+            "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_rounded_op", "sqd_interval_op", "sqd_last_error"]
 
 
@@ -247,15 +249,17 @@ class Core:
 
     def prove(self, critical_capacity=1 << 16):
         c = Counts()
-        cl = np.zeros((self.N, critical_capacity))
-        cu = np.zeros((self.N, critical_capacity))
         k = _SZ()
         secs = ctypes.c_double()
-        _check(lib().sqd_prove(self._h, ctypes.byref(c), _dptr(cl), _dptr(cu), critical_capacity, ctypes.byref(k),
-                                ctypes.byref(secs)))
+        _check(lib().sqd_prove(self._h, ctypes.byref(c), None, None, 0, ctypes.byref(k), ctypes.byref(secs)))
         m = min(k.value, critical_capacity)
-        return {"counts": c.as_dict(), "crit_lb": cl[:, :m].copy(), "crit_ub": cu[:, :m].copy(),
-                "n_critical": int(k.value), "seconds": secs.value}
+        cl = np.zeros((self.N, m))
+        cu = np.zeros((self.N, m))
+        if m:
+            n = _SZ()
+            _check(lib().sqd_core_criticals(self._h, _dptr(cl), _dptr(cu), m, ctypes.byref(n)))
+        return {"counts": c.as_dict(), "crit_lb": cl, "crit_ub": cu, "n_critical": int(k.value),
+                "seconds": secs.value}
 
 
     def set_profiling(self, enable=True, only=None):
