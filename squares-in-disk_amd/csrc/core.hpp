@@ -75,7 +75,8 @@ struct LevelArgs {
     const unsigned long long* in_base;   // first input entry (nullptr: 0)
     const unsigned long long* in_count;  // number of inputs
     unsigned long long* hull;            // NONFINAL last level: [origin][arg][4] keys
-    DeviceCounters* counters;
+    DeviceCounters* counters;            // overflow flag
+    unsigned long long* block_counts;    // per-block count slots (block_counts_add)
     const double* consts;
     unsigned generations;
 };
@@ -181,11 +182,34 @@ __device__ __forceinline__ unsigned long long wave_append(bool flag, unsigned lo
     base = __shfl(base, first);
     return base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
 }
-__device__ __forceinline__ void wave_add(unsigned long long v, unsigned long long* counter) {
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    const unsigned long long active = __ballot(1);
-    const int first = __ffsll(static_cast<long long>(active)) - 1;
-    if(__lane_id() == first && v) atomicAdd(counter, v);
+    return v;
+}
+// Adds the block's cuboid, leaf and critical counts to the block's own slot of
+// `slots` (4 words per block index).  Device-scope atomics on one address from
+// all XCDs serialise, so the counts are not accumulated in one place on the
+// device: blocks of one launch own distinct slots and launches on a stream do
+// not overlap, so a plain read-modify-write is exact, and the host sums the
+// slots.  Every thread of the block calls this once, at the end of the kernel.
+__device__ inline void block_counts_add(unsigned long long* slots, unsigned long long cub, unsigned long long leaf,
+                                        unsigned long long crit) {
+    __shared__ unsigned long long part[3][kBlock / 64];
+    cub = wave_sum(cub);
+    leaf = wave_sum(leaf);
+    crit = wave_sum(crit);
+    const unsigned w = threadIdx.x / 64u;
+    if(__lane_id() == 0) { part[0][w] = cub; part[1][w] = leaf; part[2][w] = crit; }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, c = 0;
+        for(unsigned i = 0; i < blockDim.x / 64u; ++i) { a += part[0][i]; b += part[1][i]; c += part[2][i]; }
+        unsigned long long* slot = slots + 4ull * blockIdx.x;
+        if(a) slot[0] += a;
+        if(b) slot[1] += b;
+        if(c) slot[2] += c;
+    }
 }
 
 template<class P>
@@ -340,9 +364,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
             }
         }
     }
-    wave_add(c_cub, &args.counters->cuboids);
-    wave_add(c_leaf, &args.counters->leaves);
-    if(MODE == FINAL) wave_add(c_crit, &args.counters->criticals);
+    block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
 }
 
 // ---------------------------------------------------------------- dynamic levels
@@ -401,8 +423,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
             }
         }
     }
-    wave_add(c_cub, &args.counters->cuboids);
-    wave_add(c_leaf, &args.counters->leaves);
+    block_counts_add(args.block_counts, c_cub, c_leaf, 0ull);
 }
 
 // ---------------------------------------------------------------- round glue
@@ -428,7 +449,7 @@ template<class P>
 __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, const unsigned long long* base_ptr,
                                                           const unsigned long long* count_ptr, unsigned long long* hull,
                                                           BoxList fin, BoxList spl, DeviceCounters* counters,
-                                                          unsigned generations) {
+                                                          unsigned long long* block_counts, unsigned generations) {
     constexpr int N = P::N;
     const unsigned long long n = *count_ptr, base = *base_ptr;
     const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -455,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, const u
             }
         }
     }
-    if(P::NSTAT == 0) wave_add(have ? 1ull : 0ull, &counters->leaves);
+    if(P::NSTAT == 0) block_counts_add(block_counts, 0ull, have ? 1ull : 0ull, 0ull);
     const bool to_final = have && depth >= generations;
     const bool to_split = have && !to_final;
     const unsigned long long pf = wave_append(to_final, fin.count);
@@ -501,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned 
 
 // Final pass without static variables: every cuboid is one critical leaf.
 template<class P>
-__global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList out, DeviceCounters* counters) {
+__global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList out, unsigned long long* block_counts) {
     const unsigned long long n = *fin.count;
     const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool v = i < n;
@@ -509,8 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList 
     if(v) load_box<P>(fin, i, lb, ub);
     const unsigned long long pos = wave_append(v, out.count);
     if(v && pos < out.cap) store_box<P>(out, pos, lb, ub);
-    wave_add(v ? 1ull : 0ull, &counters->leaves);
-    wave_add(v ? 1ull : 0ull, &counters->criticals);
+    block_counts_add(block_counts, 0ull, v ? 1ull : 0ull, v ? 1ull : 0ull);
 }
 
 // Initial runtime box: outward compile-time box, then 50*N rounds of all bounds.

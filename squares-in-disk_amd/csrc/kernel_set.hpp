@@ -26,8 +26,10 @@ struct KernelSet {
     // Hulls of the round's items [*base, *base + *count) (at most max_n of them).
     hipError_t (*hull_finalize)(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
                                 unsigned long long max_n, unsigned long long* hull, dev::BoxList fin,
-                                dev::BoxList spl, dev::DeviceCounters* c, unsigned gens, hipStream_t s);
-    hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s);
+                                dev::BoxList spl, dev::DeviceCounters* c, unsigned long long* block_counts,
+                                unsigned gens, hipStream_t s);
+    hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, unsigned long long* block_counts, unsigned grid,
+                                 hipStream_t s);
     hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out,
                               dev::BoxList list, unsigned long long* zero, unsigned zero_words, hipStream_t s);
     hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
@@ -104,16 +106,17 @@ hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned 
 template<class P>
 hipError_t hull_finalize(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
                          unsigned long long max_n, unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
-                         dev::DeviceCounters* c, unsigned gens, hipStream_t s) {
+                         dev::DeviceCounters* c, unsigned long long* block_counts, unsigned gens, hipStream_t s) {
     if(max_n == 0) return hipSuccess;
     unsigned grid = static_cast<unsigned>((max_n + dev::kBlock - 1) / dev::kBlock);
     hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, base, count, hull, fin,
-                       spl, c, gens);
+                       spl, c, block_counts, gens);
     return hipGetLastError();
 }
 template<class P>
-hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, dev::DeviceCounters* c, unsigned grid, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_final_nostatic<P>), dim3(grid), dim3(dev::kBlock), 0, s, fin, out, c);
+hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, unsigned long long* block_counts, unsigned grid,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_final_nostatic<P>), dim3(grid), dim3(dev::kBlock), 0, s, fin, out, block_counts);
     return hipGetLastError();
 }
 template<class P>

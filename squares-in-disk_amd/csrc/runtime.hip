@@ -75,17 +75,10 @@ static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
-// Start of a proof (or of a host-driven call): counters, stack and critical
-// counts, and every level counter to zero.
-__global__ void k_state_reset(DeviceState* st) {
-    const unsigned t = threadIdx.x;
-    if(blockIdx.x != 0) return;
-    if(t == 0) {
-        st->counters = dev::DeviceCounters{0, 0, 0, 0};
-        st->stack_count = st->fin_count = st->spl_count = st->crit_count = st->scratch_count = 0;
-        st->round_k = st->round_base = st->host_n = st->frontier_full = 0;
-    }
-    if(t < 64) st->level_count[t] = 0;
+// Start of a proof (or of a host-driven call): the whole control block (state
+// and per-block count slots) to zero.
+__global__ void k_state_reset(unsigned long long* words, unsigned long long n_words) {
+    for(unsigned long long i = threadIdx.x; blockIdx.x == 0 && i < n_words; i += blockDim.x) words[i] = 0ull;
 }
 // Start of a round: pop K = min(k_max, S, room / (fanout - 1)) cuboids from
 // the top of the stack (their slots are reused by the children the round
@@ -110,6 +103,35 @@ __global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigne
         st->spl_count = 0;
     }
     if(t < 64) st->level_count[t] = 0;
+}
+// Adds the per-block count slots into the state's counters and clears them
+// (one block; run before the state is read).
+__global__ void k_fold_counts(DeviceState* st, unsigned long long* slots, unsigned long long n_slots) {
+    __shared__ unsigned long long part[3][256];
+    unsigned long long a = 0, b = 0, c = 0;
+    for(unsigned long long i = threadIdx.x; i < n_slots; i += blockDim.x) {
+        a += slots[4 * i + 0];
+        b += slots[4 * i + 1];
+        c += slots[4 * i + 2];
+        slots[4 * i + 0] = slots[4 * i + 1] = slots[4 * i + 2] = 0ull;
+    }
+    part[0][threadIdx.x] = a;
+    part[1][threadIdx.x] = b;
+    part[2][threadIdx.x] = c;
+    __syncthreads();
+    for(unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if(threadIdx.x < w) {
+            part[0][threadIdx.x] += part[0][threadIdx.x + w];
+            part[1][threadIdx.x] += part[1][threadIdx.x + w];
+            part[2][threadIdx.x] += part[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) {
+        st->counters.cuboids += part[0][0];
+        st->counters.leaves += part[1][0];
+        st->counters.criticals += part[2][0];
+    }
 }
 // Level counters to zero (between chunks of a host-driven call).
 __global__ void k_levels_reset(DeviceState* st) {
@@ -194,10 +216,13 @@ struct sqd_core {
     double* d_consts = nullptr;
     double* d_box_ct = nullptr;     // 2N: outward compile-time box
     double* d_box_init = nullptr;   // 2N: initial runtime box
-    sqd::DeviceState* d_state = nullptr;
-    sqd::DeviceState* h_state = nullptr;   // pinned host copy
+    sqd::DeviceState* d_state = nullptr;   // followed by n_slots count slots of 4 words
+    sqd::DeviceState* h_state = nullptr;   // pinned host copy of the same block
+    unsigned long long* d_slots = nullptr;
+    unsigned long long n_slots = 0;        // >= the largest grid of a counting kernel
+    std::size_t state_bytes = 0;
     unsigned long long* d_hull = nullptr;
-    bool hull_dirty = true;                // hull records need a full reset before use
+    bool dirty = true;                // hull records and count slots need a full reset before use
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
     unsigned long long k_max = 0;    // dynamic cuboids per round
@@ -288,13 +313,18 @@ void set_count(sqd_core* c, unsigned long long* p, unsigned long long v) {
 }
 void reset_state(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_state_reset", [&] {
-        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, c->stream, c->d_state);
+        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(256), 0, c->stream, reinterpret_cast<unsigned long long*>(c->d_state),
+                           static_cast<unsigned long long>(c->state_bytes / sizeof(unsigned long long)));
         return hipGetLastError();
     });
 }
-// Brings the whole device state to the pinned host copy (one copy, one sync)
-// and checks the overflow flags.
+// Folds the count slots into the counters and brings the device state to the
+// pinned host copy (one copy, one sync); checks the overflow flags.
 const DeviceState& read_state(sqd_core* c) {
+    launch(c, KIND_SET_COUNT, "k_fold_counts", [&] {
+        hipLaunchKernelGGL(k_fold_counts, dim3(1), dim3(256), 0, c->stream, c->d_state, c->d_slots, c->n_slots);
+        return hipGetLastError();
+    });
     hip_check(hipMemcpyAsync(c->h_state, c->d_state, sizeof(DeviceState), hipMemcpyDeviceToHost, c->stream),
               "copy state");
     sync(c);
@@ -322,6 +352,7 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.in_count = count;
     a.hull = c->d_hull;
     a.counters = &c->d_state->counters;
+    a.block_counts = c->d_slots;
     a.consts = c->d_consts;
     a.generations = c->settings.generation_count;
     return a;
@@ -407,16 +438,21 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
     sync(c);
 }
 
-// Hull records are cleared by the kernels that consume them; after a failed
-// call they are reset in full before the next use.
-void prepare_hull(sqd_core* c) {
-    if(c->hull_dirty && c->ks->NSTAT > 0) {
-        launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
-            hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull, c->k_max * static_cast<unsigned long long>(c->N));
-            return hipGetLastError();
-        });
+// Hull records and count slots are cleared by the kernels that consume them
+// (hull finalize / decode, the count fold).  Every computing entry point starts
+// with begin_call and marks the core clean when it completes; after a failed
+// call both are reset in full before the next use.
+void begin_call(sqd_core* c) {
+    if(c->dirty) {
+        if(c->ks->NSTAT > 0)
+            launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
+                hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull,
+                                   c->k_max * static_cast<unsigned long long>(c->N));
+                return hipGetLastError();
+            });
+        reset_state(c);
     }
-    c->hull_dirty = true;   // until the call completes
+    c->dirty = true;   // until the call completes
 }
 void levels_reset(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_levels_reset", [&] {
@@ -461,11 +497,11 @@ void enqueue_round(sqd_core* c) {
     if(c->ks->NSTAT > 0) run_static(c, dev::NONFINAL, items, &st->round_base, &st->round_k, K, c->crit.view());
     launch(c, KIND_HULL_FINALIZE, "hull finalize", [&] {
         return c->ks->hull_finalize(items, &st->round_base, &st->round_k, K, c->d_hull, c->fin.view(), c->spl.view(),
-                                    &st->counters, c->settings.generation_count, c->stream);
+                                    &st->counters, c->d_slots, c->settings.generation_count, c->stream);
     });
     if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, K, c->crit.view());
     else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
-        return c->ks->final_nostatic(c->fin.view(), c->crit.view(), &st->counters,
+        return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_slots,
                                      static_cast<unsigned>((K + 255) / 256), c->stream);
     });
     run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, K, c->stack.view());
@@ -584,9 +620,16 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         kmax = std::min(kmax, lvl_cap / worst_split);
         if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
         c->k_max = kmax;
-        c->d_state = dalloc<DeviceState>(1);
+        // count slots: one per block of the largest counting grid (level kernels
+        // run at most 8 blocks of 256 threads per CU; the small per-round
+        // kernels need at most k_max / 256 + 1 blocks)
+        c->n_slots = std::max<unsigned long long>(8ull * c->cus, kmax / 256 + 2);
+        if(c->grid_override > c->n_slots) c->grid_override = static_cast<unsigned>(c->n_slots);
+        c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * 4 * sizeof(unsigned long long);
+        c->d_state = static_cast<DeviceState*>(static_cast<void*>(dalloc<unsigned long long>(c->state_bytes / 8)));
+        c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
         void* hs = nullptr;
-        hip_check(hipHostMalloc(&hs, sizeof(DeviceState), hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipHostMalloc(&hs, c->state_bytes, hipHostMallocDefault), "hipHostMalloc");
         c->h_state = static_cast<DeviceState*>(hs);
         DeviceState* st = c->d_state;
         c->stack.create(c->N, std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1), true, false,
@@ -641,10 +684,12 @@ int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
+        begin_call(c);
         enqueue_initial_queue(c);
         const DeviceState& h = read_state(c);
         const unsigned long long len = h.stack_count;
         const sqd_counts r = counts_of(h);
+        c->dirty = false;
         *n = len;
         if(lb && ub) download_soa(c, c->stack, std::min<unsigned long long>(len, capacity), lb, ub, capacity);
         if(counts) *counts = r;
@@ -656,8 +701,8 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
     return guarded([&] {
         if(!c || (n && (!lb || !ub || !hull_lb || !hull_ub || !has_hull))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
+        begin_call(c);
         reset_state(c);
-        prepare_hull(c);
         std::vector<double> clb, cub, hlb, hub;
         for(std::size_t off = 0; off < n; off += c->k_max) {
             const std::size_t m = std::min<std::size_t>(c->k_max, n - off);
@@ -685,7 +730,7 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
                 }
         }
         sqd_counts r = counts_of(read_state(c));
-        c->hull_dirty = false;
+        c->dirty = false;
         if(c->ks->NSTAT == 0) r.num_leaf_cuboids += n;  // without static variables each box is its own leaf
         if(counts) *counts = r;
     });
@@ -697,16 +742,18 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
         if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         if(n > c->fin.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one final pass");
+        begin_call(c);
         reset_state(c);
         upload_soa(c, c->fin, lb, ub, n, 0);
         if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, n, c->crit.view());
         else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
-            return c->ks->final_nostatic(c->fin.view(), c->crit.view(), &c->d_state->counters,
+            return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_slots,
                                          static_cast<unsigned>((n + 255) / 256 + 1), c->stream);
         });
         const DeviceState& h = read_state(c);
         const unsigned long long k = h.crit_count;
         const sqd_counts r = counts_of(h);
+        c->dirty = false;
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
         if(counts) *counts = r;
@@ -719,12 +766,14 @@ int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
         if(!c || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         if(n > c->spl.cap) throw Failure(SQD_ERR_CAPACITY, "too many boxes for one split");
+        begin_call(c);
         reset_state(c);
         upload_soa(c, c->spl, lb, ub, n, 1);
         run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, n, c->stack.view());
         const DeviceState& h = read_state(c);
         const unsigned long long k = h.stack_count;
         const sqd_counts r = counts_of(h);
+        c->dirty = false;
         if(n_out) *n_out = k;
         if(out_lb && out_ub) download_soa(c, c->stack, std::min<unsigned long long>(k, capacity), out_lb, out_ub, capacity);
         if(counts) *counts = r;
@@ -737,7 +786,7 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
         if(!c) throw std::invalid_argument("null core");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto t0 = std::chrono::steady_clock::now();
-        prepare_hull(c);
+        begin_call(c);
         enqueue_initial_queue(c);
         // One host round trip per round: the round's size is decided on the
         // device, the host only learns whether the frontier is empty.
@@ -746,7 +795,7 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
             enqueue_round(c);
             h = &read_state(c);
         } while(h->stack_count > 0);
-        c->hull_dirty = false;
+        c->dirty = false;
         const sqd_counts r = counts_of(*h);
         const unsigned long long k = h->crit_count;
         auto t1 = std::chrono::steady_clock::now();
