@@ -128,6 +128,13 @@ int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_
  * to its on_critical callback (prover/proof_driver.hpp:499-509). */
 int sqd_core_criticals(sqd_core* core, double* crit_lb, double* crit_ub, size_t capacity, size_t* n);
 
+/* Evaluations of acos by this core's program whose rounding direction could not
+ * be decided at 2^-113 precision, so that the interval was widened by one double
+ * (still an enclosure); counted since the library was loaded.  Zero means every
+ * acos was rounded exactly like the reference's MPFR evaluation.  No counterpart
+ * in the reference. */
+int sqd_core_acos_fallbacks(sqd_core* core, uint64_t* n);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Device time per kernel kind, from HIP events recorded on the core's stream
@@ -152,15 +159,21 @@ int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t cap
 /* ---------------------------------------------------------------- tests */
 
 /* Elementwise directed rounding on the device (op: 0 add, 1 sub, 2 mul, 3 div,
- * 4 sqrt): lo = RD(a op b), hi = RU(a op b).  Arithmetic parity checks. */
+ * 4 sqrt(a), 5 acos(a)): lo = RD(a op b), hi = RU(a op b).  Arithmetic parity
+ * checks (acos: ivarp/src/interval_acos.cpp:29-66 rounds with MPFR). */
 int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi);
 
 /* Interval operations on the device (run-time sign decisions, as for operands
  * without compile-time bounds): n operand pairs of 3 doubles (lb, ub, undefined
  * flag 0/1), op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 square(a), 6 -a,
- * 7 max, 8 min, 9 splitter point (a = (lb, ub, -), b = (i, n, -) -> out lb).
+ * 7 max, 8 min, 9 splitter point (a = (lb, ub, -), b = (i, n, -) -> out lb),
+ * 10 acos(a) with domain clamping to [-1, 1].
  * Interval parity checks (ivarp/test/interval_*.cpp, splitter.cpp). */
 int sqd_interval_op(int device, int op, size_t n, const double* a, const double* b, double* out);
+
+/* Evaluations of acos in the test kernels whose rounding direction could not be
+ * decided at 2^-113 precision (results widened instead); 0 in every run so far. */
+int sqd_test_acos_fallbacks(int device, uint64_t* n);
 
 const char* sqd_last_error(void);
 

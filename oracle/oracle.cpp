@@ -18,6 +18,8 @@
 #include <thread>
 #include <vector>
 
+#include <mpfr.h>
+
 namespace {
 
 // ---------------------------------------------------------------- arithmetic
@@ -198,6 +200,27 @@ IV interval_sqrt(const IV& x, CT B) {
     }
     return IV{rd_sqrt(lb), ru_sqrt(ub), und};
 }
+// acos rounded down / up, correctly (MPFR at 53 bits), like the reference's
+// cpu_acos_rd / cpu_acos_ru (ivarp/src/interval_acos.cpp:29-66).  NaN outside [-1, 1].
+double acos_dir(double x, bool up) {
+    count(1);
+    mpfr_t m;
+    mpfr_init2(m, 53);
+    mpfr_set_d(m, x, MPFR_RNDN);
+    mpfr_acos(m, m, up ? MPFR_RNDU : MPFR_RNDD);
+    const double r = mpfr_get_d(m, up ? MPFR_RNDU : MPFR_RNDD);
+    mpfr_clear(m);
+    return r;
+}
+// fn_domain_handle_bounds (math_fn/interval_asin.hpp:13-27) + interval_acos (interval_acos.hpp:19-29)
+IV interval_acos(IV x, CT B) {
+    constexpr std::int64_t kOne = 1000000;
+    bool und = x.und;
+    if(B.lb < -kOne && x.lb < -1.0) { x.lb = -1.0; und = true; }
+    if(B.ub > kOne && x.ub > 1.0) { x.ub = 1.0; und = true; }
+    return IV{acos_dir(x.ub, false), acos_dir(x.lb, true), und};
+}
+
 inline double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }
 inline double dmin(double n1, double n2) { return n1 < n2 ? n1 : n2; }
 
@@ -307,7 +330,6 @@ Program parse(const std::string& text) {
         collect(p, c, o, seen);
         p.pred_order[static_cast<std::size_t>(c)] = o;
     }
-    for(const auto& n : p.nodes) if(n.op == C_ACOS) throw std::runtime_error("acos is not supported by the oracle yet");
     return p;
 }
 
@@ -338,6 +360,7 @@ struct Evaluator {
             case C_MUL: r = mul_check_signs(K(0), K(1), KCT(0), KCT(1)); return;
             case C_DIV: r = interval_div(K(0), K(1), KCT(0), KCT(1)); return;
             case C_SQRT: r = interval_sqrt(K(0), KCT(0)); return;
+            case C_ACOS: r = interval_acos(K(0), KCT(0)); return;
             case C_SQUARE: {
                 IV x = mul_check_signs(K(0), K(0), kUnbounded, kUnbounded);
                 if(x.lb < 0) x.lb = 0;
@@ -914,6 +937,7 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
                 case 2: lo[i] = rd_mul(a[i], b[i]); hi[i] = ru_mul(a[i], b[i]); break;
                 case 3: lo[i] = rd_div(a[i], b[i]); hi[i] = ru_div(a[i], b[i]); break;
                 case 4: lo[i] = rd_sqrt(a[i]); hi[i] = ru_sqrt(a[i]); break;
+                case 5: lo[i] = acos_dir(a[i], false); hi[i] = acos_dir(a[i], true); break;
                 default: throw std::invalid_argument("unknown op");
             }
         }
@@ -922,7 +946,7 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
 
 int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b, double* out) {
     return guarded([&] {
-        if(op < 0 || op > 9) throw std::invalid_argument("unknown interval op");
+        if(op < 0 || op > 10) throw std::invalid_argument("unknown interval op");
         RoundDownGuard g;
         for(std::uint64_t i = 0; i < n; ++i) {
             const IV x{a[3 * i], a[3 * i + 1], a[3 * i + 2] != 0.0};
@@ -943,6 +967,7 @@ int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b
                     r.lb = sp.point(static_cast<int>(y.lb));
                     break;
                 }
+                case 10: r = interval_acos(x, kUnbounded); break;
                 default: break;
             }
             out[3 * i] = r.lb;

@@ -6,7 +6,8 @@
 // the CPU baseline; nothing in the product links, loads or calls it.
 //
 // Restated reference functions (file:line):
-//   interval + - * / sqrt, min/max, comparisons, IBool
+//   interval + - * / sqrt acos, min/max, comparisons, IBool
+//        math_fn/interval_acos.hpp:19-29 with ivarp/src/interval_acos.cpp:29-66 (MPFR),
 //        ivarp/include/ivarp/number/float_interval_addsub.hpp:53-102,
 //        float_interval_mul.hpp:33-166, float_interval_div.hpp:45-107, interval_div.hpp:57-288,
 //        math_fn/interval_sqrt.hpp:72-116, number/minmax.hpp:59-177,
@@ -67,14 +68,14 @@ int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* 
                  double* out_lb, double* out_ub, std::uint64_t cap, std::uint64_t* n_out, OracleCounts* counts);
 
 // Elementwise directed-rounding primitives for arithmetic parity tests.
-// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt.  out_lo = RD(a op b), out_hi = RU(a op b).
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 acos(a).  out_lo = RD(a op b), out_hi = RU(a op b).
 int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* out_lo, double* out_hi);
 
 // Interval operations on n operand pairs, 3 doubles per interval (lb, ub,
 // undefined flag 0/1), without compile-time bound information (run-time sign
 // decisions only).  op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (of a), 5 square (of
 // a), 6 negation (of a), 7 max, 8 min, 9 splitter point: a = (lb, ub, -),
-// b = (i, n, -) -> out lb = point i of n.
+// b = (i, n, -) -> out lb = point i of n, 10 acos (of a) clamped to [-1, 1].
 int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b, double* out);
 
 // Binary64 arithmetic operations (add/sub/mul/div/sqrt) executed since the last reset.

@@ -167,7 +167,7 @@ class Oracle:
 
 
 def rounded_op(op, a, b):
-    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}[op]
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5}[op]
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     lo = np.zeros_like(a)
@@ -177,7 +177,7 @@ def rounded_op(op, a, b):
 
 
 INTERVAL_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "neg": 6, "max": 7, "min": 8,
-                "split": 9}
+                "split": 9, "acos": 10}
 
 
 def interval_op(op, a, b=None):

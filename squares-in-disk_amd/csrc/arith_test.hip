@@ -20,6 +20,7 @@ __global__ void k_rounded_op(int op, unsigned long long n, const double* a, cons
             case 2: l = rd_mul(x, y); h = ru_mul(x, y); break;
             case 3: l = rd_div(x, y); h = ru_div(x, y); break;
             case 4: l = rd_sqrt(x); h = ru_sqrt(x); break;
+            case 5: { bool hard; acosfx::acos_directed(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
             default: break;
         }
         lo[i] = l;
@@ -49,6 +50,7 @@ __global__ void k_interval_op(int op, unsigned long long n, const double* a, con
                 r.lb = split_point(x.lb, x.ub, split_width(x.lb, x.ub, cnt), static_cast<int>(y.lb), cnt);
                 break;
             }
+            case 10: r = iv_acos<true, true>(x); break;
             default: break;
         }
         out[3 * i] = r.lb;
@@ -59,7 +61,7 @@ __global__ void k_interval_op(int op, unsigned long long n, const double* a, con
 }
 
 extern "C" int sqd_interval_op(int device, int op, size_t n, const double* a, const double* b, double* out) {
-    if(op < 0 || op > 9 || !a || !b || !out) return SQD_ERR_ARGUMENT;
+    if(op < 0 || op > 10 || !a || !b || !out) return SQD_ERR_ARGUMENT;
     if(n == 0) return SQD_OK;
     double *da = nullptr, *db = nullptr, *dout = nullptr;
     const size_t bytes = 3 * n * sizeof(double);
@@ -80,7 +82,7 @@ extern "C" int sqd_interval_op(int device, int op, size_t n, const double* a, co
 }
 
 extern "C" int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi) {
-    if(op < 0 || op > 4 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
+    if(op < 0 || op > 5 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
     if(n == 0) return SQD_OK;
     double *da = nullptr, *db = nullptr, *dl = nullptr, *dh = nullptr;
     const size_t bytes = n * sizeof(double);
@@ -99,5 +101,14 @@ extern "C" int sqd_rounded_op(int device, int op, size_t n, const double* a, con
     if(e == hipSuccess) e = hipMemcpy(lo, dl, bytes, hipMemcpyDeviceToHost);
     if(e == hipSuccess) e = hipMemcpy(hi, dh, bytes, hipMemcpyDeviceToHost);
     for(double* p : {da, db, dl, dh}) if(p) (void)hipFree(p);
+    return e == hipSuccess ? SQD_OK : SQD_ERR_DEVICE;
+}
+
+extern "C" int sqd_test_acos_fallbacks(int device, uint64_t* n) {
+    if(!n) return SQD_ERR_ARGUMENT;
+    unsigned long long v = 0;
+    hipError_t e = hipSetDevice(device);
+    if(e == hipSuccess) e = hipMemcpyFromSymbol(&v, HIP_SYMBOL(sqd::dev::g_acos_hard), sizeof v, 0, hipMemcpyDeviceToHost);
+    *n = v;
     return e == hipSuccess ? SQD_OK : SQD_ERR_DEVICE;
 }

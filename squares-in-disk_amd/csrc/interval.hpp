@@ -14,6 +14,7 @@
 // compile-time bounds decide, Tri::R when the runtime value decides.
 #pragma once
 #include "rounding.hpp"
+#include "acos_fixed.hpp"
 
 namespace sqd {
 namespace dev {
@@ -159,6 +160,26 @@ __device__ inline IV iv_sqrt(IV x) {
         und = true;
     }
     return IV{rd_sqrt(lb), ru_sqrt(ub), und};
+}
+
+// Evaluations of acos whose rounding direction could not be decided (see
+// acos_fixed.hpp); one counter per compiled program, read by the host.
+static __device__ unsigned long long g_acos_hard = 0;
+
+// acos with the reference's domain handling (math_fn/interval_asin.hpp:13-27,
+// interval_acos.hpp:19-29): where the argument's compile-time bounds allow values
+// outside [-1, 1] the interval is clamped and marked possibly undefined.
+template<bool CheckLb, bool CheckUb>
+__device__ inline IV iv_acos(IV x) {
+    bool und = x.und;
+    if(CheckLb && x.lb < -1.0) { x.lb = -1.0; und = true; }
+    if(CheckUb && x.ub > 1.0) { x.ub = 1.0; und = true; }
+    double l1, h1, l2, h2;
+    bool hard1, hard2;
+    acosfx::acos_directed(x.ub, &l1, &h1, &hard1);
+    acosfx::acos_directed(x.lb, &l2, &h2, &hard2);
+    if(hard1 || hard2) atomicAdd(&g_acos_hard, 1ull);
+    return IV{l1, h2, und};
 }
 
 __device__ __forceinline__ double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }

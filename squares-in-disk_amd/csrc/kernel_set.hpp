@@ -34,6 +34,8 @@ struct KernelSet {
                               dev::BoxList list, unsigned long long* zero, unsigned zero_words, hipStream_t s);
     hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
                               dev::BoxList out, unsigned char* has, hipStream_t s);
+    // Undecided acos roundings of this program so far (device-wide counter).
+    hipError_t (*acos_hard)(unsigned long long* out);
 };
 
 void register_kernel_set(const KernelSet* ks);
@@ -136,6 +138,10 @@ hipError_t hull_decode(dev::BoxList items, unsigned long long n, unsigned long l
     return hipGetLastError();
 }
 
+inline hipError_t acos_hard(unsigned long long* out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(dev::g_acos_hard), sizeof(unsigned long long), 0, hipMemcpyDeviceToHost);
+}
+
 } // namespace detail
 
 template<class P>
@@ -155,6 +161,7 @@ KernelSet make_kernel_set(std::uint64_t fingerprint, const char* key, const doub
     k.final_nostatic = &detail::final_nostatic<P>;
     k.initial_box = &detail::initial_box<P>;
     k.hull_decode = &detail::hull_decode<P>;
+    k.acos_hard = &detail::acos_hard;
     return k;
 }
 

@@ -817,6 +817,17 @@ int sqd_core_criticals(sqd_core* c, double* crit_lb, double* crit_ub, size_t cap
     });
 }
 
+int sqd_core_acos_fallbacks(sqd_core* c, uint64_t* n) {
+    return guarded([&] {
+        if(!c || !n) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        unsigned long long v = 0;
+        hip_check(c->ks->acos_hard(&v), "acos counter");
+        *n = v;
+    });
+}
+
 int sqd_core_set_profiling(sqd_core* c, int enable, int only_mode, int only_level) {
     return guarded([&] {
         if(!c) throw std::invalid_argument("null core");

@@ -158,7 +158,13 @@ struct Emitter {
                 break;
             }
             case FOp::Ite: case FOp::Or: case FOp::And: throw std::logic_error("lazy node emitted as plain code");
-            case FOp::Acos: throw std::runtime_error("acos is not supported by the GPU code generator yet");
+            case FOp::Acos: {
+                // domain checks only where the compile-time bounds leave [-1, 1]
+                const bool chk_lb = KN(0).lb < fpb::int_to_fp(-1), chk_ub = KN(0).ub > fpb::int_to_fp(1);
+                o << "const IV " << v(id) << " = iv_acos<" << (chk_lb ? "true" : "false") << ", "
+                  << (chk_ub ? "true" : "false") << ">(" << v(K(0)) << ");";
+                break;
+            }
             case FOp::BoolConst: o << "const IB " << b(id) << "{" << (n.b_def ? "true" : "false") << ", " << (n.b_poss ? "true" : "false") << "};"; break;
             case FOp::Lt: o << "const IB " << b(id) << " = ib_lt(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
             case FOp::Le: o << "const IB " << b(id) << " = ib_le(" << v(K(0)) << ", " << v(K(1)) << ");"; break;
@@ -174,10 +180,7 @@ struct Emitter {
 
 } // namespace
 
-bool supported(const flat::Program& p) {
-    for(const auto& n : p.nodes) if(n.op == FOp::Acos) return false;
-    return true;
-}
+bool supported(const flat::Program&) { return true; }
 
 std::string generate(const flat::Program& p, const std::string& key) {
     Emitter e{p, {}, {}};

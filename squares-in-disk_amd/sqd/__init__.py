@@ -86,6 +86,8 @@ def lib():
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
             "sqd_interval_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
+            "sqd_test_acos_fallbacks": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_core_acos_fallbacks": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_last_error": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -103,7 +105,8 @@ def exported_symbols():
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_criticals",
             "sqd_core_set_profiling",
-            "sqd_core_kernel_stats", "sqd_rounded_op", "sqd_interval_op", "sqd_last_error"]
+            "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_rounded_op", "sqd_interval_op",
+            "sqd_test_acos_fallbacks", "sqd_last_error"]
 
 
 def _check(rc):
@@ -262,6 +265,12 @@ class Core:
                 "seconds": secs.value}
 
 
+    def acos_fallbacks(self):
+        """Undecided acos roundings of this core's program (results widened by one double)."""
+        n = ctypes.c_uint64()
+        _check(lib().sqd_core_acos_fallbacks(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def set_profiling(self, enable=True, only=None):
         """Time launches with HIP events on the core's stream (clears the statistics).
 
@@ -280,7 +289,7 @@ class Core:
 
 def rounded_op(op, a, b, device=0):
     """Device RD/RU of a op b (op: 'add', 'sub', 'mul', 'div', 'sqrt')."""
-    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}[op]
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5}[op]
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     lo = np.zeros_like(a)
@@ -289,8 +298,15 @@ def rounded_op(op, a, b, device=0):
     return lo, hi
 
 
+def acos_fallbacks(device=0):
+    """Undecided acos roundings in the test kernels since the library was loaded."""
+    n = ctypes.c_uint64()
+    _check(lib().sqd_test_acos_fallbacks(device, ctypes.byref(n)))
+    return int(n.value)
+
+
 INTERVAL_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "neg": 6, "max": 7, "min": 8,
-                "split": 9}
+                "split": 9, "acos": 10}
 
 
 def interval_op(op, a, b=None, device=0):

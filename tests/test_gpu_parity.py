@@ -10,7 +10,7 @@ from conftest import PUBLISHED
 
 pytestmark = pytest.mark.gpu
 
-GPU_LEMMAS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven"]
+GPU_LEMMAS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven", "more1", "more2"]
 
 
 def _bits(a):
@@ -46,6 +46,41 @@ def test_rounded_ops_bit_exact(op):
     assert np.array_equal(_bits(ghi)[~np.isnan(chi)], _bits(chi)[~np.isnan(chi)])
 
 
+def _acos_inputs(seed, n):
+    rng = np.random.default_rng(seed)
+    x = np.concatenate([
+        rng.uniform(-1, 1, n),
+        1 - np.ldexp(rng.integers(1, 1 << 20, n).astype(float), -53),            # just below 1
+        -1 + np.ldexp(rng.integers(1, 1 << 20, n // 4).astype(float), -53),      # just above -1
+        # 1 - k^2/2 * 2^(2j-53): sqrt(2 (1 - x)) is a short dyadic number (hardest to round)
+        1 - np.ldexp(rng.integers(1, 64, 4096).astype(float) ** 2 / 2, -53 + 2 * rng.integers(0, 20, 4096)),
+        np.ldexp(rng.uniform(0.5, 1, n), -rng.integers(0, 200, n)),              # tiny magnitudes
+        [0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 5e-324, -5e-324, 2.2250738585072014e-308]])
+    return x[(x >= -1) & (x <= 1)]
+
+
+def test_acos_bit_exact_with_mpfr():
+    # the reference rounds acos with MPFR (ivarp/src/interval_acos.cpp:29-66); the oracle calls MPFR
+    x = _acos_inputs(5, 1 << 18)
+    before = sqd.acos_fallbacks()
+    glo, ghi = sqd.rounded_op("acos", x, x)
+    olo, ohi = pyoracle.rounded_op("acos", x, x)
+    assert np.array_equal(_bits(glo), _bits(olo)) and np.array_equal(_bits(ghi), _bits(ohi))
+    assert sqd.acos_fallbacks() == before        # every direction decided at 2^-113
+    out = sqd.rounded_op("acos", np.array([1.5, -2.0, np.inf, np.nan]), np.zeros(4))
+    assert np.all(np.isnan(out[0])) and np.all(np.isnan(out[1]))
+
+
+def test_acos_interval_domain_clamp():
+    # interval_asin.hpp:13-27: with unbounded compile-time bounds the argument is clamped to
+    # [-1, 1] and marked possibly undefined
+    A = np.array([[-2.0, 0.5, 0.0], [-0.25, 3.0, 0.0], [0.1, 0.2, 0.0], [0.3, 0.4, 1.0]])
+    g = sqd.interval_op("acos", A)
+    o = pyoracle.interval_op("acos", A)
+    assert np.array_equal(_bits(g), _bits(o))
+    assert list(g[:, 2]) == [1.0, 1.0, 0.0, 1.0]
+
+
 @pytest.fixture(scope="module")
 def cores():
     made = {}
@@ -73,7 +108,7 @@ def _sorted_boxes(lb, ub):
     return m[np.lexsort(m.T[::-1])]
 
 
-@pytest.mark.parametrize("key", ["one", "two", "three", "five", "ec2"])
+@pytest.mark.parametrize("key", ["one", "two", "three", "five", "more2", "ec2"])
 def test_initial_queue(cores, program_text, key):
     glb, gub, gc = cores(key).initial_queue()
     olb, oub, oc = pyoracle.Oracle(program_text(key)).initial_queue()
@@ -82,7 +117,7 @@ def test_initial_queue(cores, program_text, key):
 
 
 @pytest.mark.parametrize("key,limit", [("one", 128), ("two", 512), ("three", 256), ("four", 128), ("five", 256),
-                                       ("six", 128), ("seven", 64), ("ec2", 16)])
+                                       ("six", 128), ("seven", 64), ("more1", 256), ("more2", 64), ("ec2", 16)])
 def test_nonfinal_hulls(cores, program_text, key, limit):
     o = pyoracle.Oracle(program_text(key))
     qlb, qub, _ = o.initial_queue()
@@ -128,10 +163,12 @@ def test_final_pass_direct(cores):
 
 @pytest.mark.parametrize("key", GPU_LEMMAS)
 def test_prove_published_counts(cores, key):
-    r = cores(key).prove()
+    core = cores(key)
+    r = core.prove()
     c = r["counts"]
     assert (c["num_cuboids"], c["num_leaf_cuboids"]) == PUBLISHED[key]
     assert c["num_critical_cuboids"] == 0 and r["n_critical"] == 0
+    assert core.acos_fallbacks() == 0
 
 
 def test_prove_expected_criticals_2d(cores, program_text):

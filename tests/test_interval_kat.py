@@ -170,3 +170,52 @@ def test_oracle_splitter():
     a = [[0.1, 1.7, 0.0]] * 9
     pts = pyoracle.interval_op("split", a, [[i, 8, 0.0] for i in range(9)])[:, 0]
     assert pts[0] == 0.1 and pts[8] == 1.7 and np.all(np.diff(pts) > 0)
+
+
+def _pi_fraction(bits=200):
+    """pi as a fraction accurate to 2^-190 (Machin: pi = 16 atan(1/5) - 4 atan(1/239))."""
+    def atan_inv(n):
+        one = 1 << bits
+        s = t = one // n
+        k, sign = 1, -1
+        while t:
+            t //= n * n
+            s += sign * (t // (2 * k + 1))
+            sign, k = -sign, k + 1
+        return s
+    return Fraction(16 * atan_inv(5) - 4 * atan_inv(239), 1 << bits)
+
+
+def test_oracle_acos_known_values():
+    # acos as in ivarp/src/interval_acos.cpp:29-66: correctly rounded down / up
+    pi = _pi_fraction()
+    eps = Fraction(1, 1 << 150)
+    cases = [(0.0, pi / 2), (-1.0, pi), (0.5, pi / 3), (-0.5, 2 * pi / 3)]
+    for x, v in cases:
+        lo, hi = pyoracle.rounded_op("acos", [x], [0.0])
+        lo, hi = float(lo[0]), float(hi[0])
+        assert Fraction(lo) < v - eps and v + eps < Fraction(hi) and math.nextafter(lo, INF) == hi, x
+    lo, hi = pyoracle.rounded_op("acos", [1.0], [0.0])
+    assert lo[0] == 0.0 and hi[0] == 0.0
+    lo, hi = pyoracle.rounded_op("acos", [1.5, -1.0000000000000002], [0.0, 0.0])
+    assert np.all(np.isnan(lo)) and np.all(np.isnan(hi))
+
+
+def test_oracle_acos_monotone_adjacent():
+    rng = np.random.default_rng(9)
+    x = np.sort(rng.uniform(-1, 1, 20000))
+    lo, hi = pyoracle.rounded_op("acos", x, x)
+    assert np.all(np.diff(lo) <= 0) and np.all(np.diff(hi) <= 0)
+    up = np.nextafter(lo, INF)
+    assert np.array_equal(up, hi)                  # acos of these is irrational: RU is RD's successor
+    ref = np.arccos(x)                              # the library result lies within one ulp
+    assert np.all(lo <= np.nextafter(ref, INF)) and np.all(hi >= np.nextafter(ref, -INF))
+
+
+def test_oracle_acos_interval_domain():
+    # fn_domain_handle_bounds (math_fn/interval_asin.hpp:13-27) with unbounded compile-time bounds
+    r = pyoracle.interval_op("acos", [[-2.0, 0.5, 0.0], [0.1, 0.2, 0.0]])
+    lo05, _ = pyoracle.rounded_op("acos", [0.5], [0.0])
+    _, hipi = pyoracle.rounded_op("acos", [-1.0], [0.0])
+    assert (r[0][0], r[0][1], r[0][2]) == (lo05[0], hipi[0], 1.0)
+    assert r[1][2] == 0.0 and r[1][0] < r[1][1]

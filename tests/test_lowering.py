@@ -64,7 +64,7 @@ def test_lowering_matches_reference_dump(program_text, key):
 S1_UB = float("1.2649110640673517")   # outward sqrt(1.6), reference dump "initial range"
 
 
-@pytest.mark.parametrize("key", ["top1", "top2", "one", "two", "three", "four"])
+@pytest.mark.parametrize("key", ["top1", "top2", "one", "two", "three", "four", "more1"])
 def test_initial_box_reference_values(program_text, key):
     lb, ub = pyoracle.Oracle(program_text(key)).initial_box()
     assert (lb[0], ub[0]) == (float("0.29499999999999998"), S1_UB)
@@ -143,10 +143,9 @@ def test_program_text_rejects_garbage():
         sqd.Program.from_text("not a program")
 
 
-def test_every_lemma_has_kernels_except_acos():
+def test_every_lemma_has_kernels():
     have = {k: sqd.Program.from_lemma(k).has_kernels for k, _ in sqd.lemmas()}
-    assert not have["more1"] and not have["more2"]     # acos is not compiled yet
-    assert all(v for k, v in have.items() if k not in ("more1", "more2"))
+    assert all(have.values()) and len(have) == 13
 
 
 def test_c_abi_exports_every_declared_function():
@@ -156,3 +155,17 @@ def test_c_abi_exports_every_declared_function():
     missing = [n for n in declared if not hasattr(lib, n)]
     assert not missing
     assert declared == set(sqd.exported_symbols())
+
+
+def test_initial_box_more2(program_text):
+    # h_i <= T_inv(s_1) (src/more_subcontainers.cpp:109-128): the upper bound is
+    # RU(RU(sqrt(RU(1 - RD(0.25 RD(0.295^2))))) - 0.295), evaluated exactly here.
+    # (The reference dump quotes 0.69406205568710399, one double lower.)
+    lb, ub = pyoracle.Oracle(program_text("more2")).initial_box()
+    assert (lb[0], ub[0]) == (float("0.29499999999999998"), S1_UB)
+    F = Fraction
+    m_hi = _ru(1 - F(_rd(F(1, 4) * F(_rd(F(lb[0]) * F(lb[0]))))))
+    t_hi = _ru(F(_sqrt_ru(m_hi)) - F(lb[0]))
+    assert t_hi == 0.6940620556871041
+    for a in range(1, len(lb)):
+        assert (lb[a], ub[a]) == (0.0, t_hi)
