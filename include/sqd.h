@@ -60,7 +60,10 @@ typedef struct sqd_settings {
     uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
     uint32_t grid_blocks;          /* blocks per level launch; default: the kernel's resident
                                       capacity (occupancy x CUs), capped by the work */
-    uint32_t reserved;
+    uint32_t fp32_prefilter;       /* nonzero: static levels first decide each child in
+                                      outward-rounded binary32 and re-evaluate only the
+                                      undecided ones in binary64 (no counterpart in the
+                                      reference, which evaluates everything in binary64) */
 } sqd_settings;
 
 /* Defaults: generation_count = 8, every sizing field 0 (chosen by the core). */
@@ -135,6 +138,31 @@ int sqd_core_criticals(sqd_core* core, double* crit_lb, double* crit_ub, size_t 
  * in the reference. */
 int sqd_core_acos_fallbacks(sqd_core* core, uint64_t* n);
 
+/* ---------------------------------------------------------------- FP32 prefilter */
+
+/* With sqd_settings.fp32_prefilter set: the static-level children of the last
+ * computing call (sqd_prove, sqd_nonfinal, sqd_final) that the binary32 pass
+ * evaluated, and those it decided (pruned) without binary64.  No counterpart in
+ * the reference. */
+int sqd_core_fp32_stats(sqd_core* core, uint64_t* checked, uint64_t* decided);
+
+/* Binary32-versus-binary64 overestimation per constraint: over n boxes (SoA,
+ * stride n) the width of the interval of (left side - right side) of each
+ * constraint that is a plain comparison, evaluated on the outward binary32 box
+ * with outward binary32 constants, divided by the same width in binary64.
+ * Entries are in the order of the program's constraint rows' roots; other
+ * constraints (boolean combinations) report 0 samples.  No counterpart in the
+ * reference. */
+typedef struct sqd_width_stat {
+    int32_t constraint;   /* node index of the constraint in the lowered program */
+    uint32_t reserved;
+    uint64_t samples;     /* boxes with finite, defined widths and a positive binary64 width */
+    double mean_ratio;    /* mean of width32 / width64 */
+    double max_ratio;
+} sqd_width_stat;
+int sqd_width_stats(sqd_core* core, const double* lb, const double* ub, size_t n, sqd_width_stat* out,
+                    size_t capacity, size_t* n_constraints);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Device time per kernel kind, from HIP events recorded on the core's stream
@@ -159,8 +187,11 @@ int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t cap
 /* ---------------------------------------------------------------- tests */
 
 /* Elementwise directed rounding on the device (op: 0 add, 1 sub, 2 mul, 3 div,
- * 4 sqrt(a), 5 acos(a)): lo = RD(a op b), hi = RU(a op b).  Arithmetic parity
- * checks (acos: ivarp/src/interval_acos.cpp:29-66 rounds with MPFR). */
+ * 4 sqrt(a), 5 acos(a)): lo = RD(a op b), hi = RU(a op b).  Ops 8..13 are the
+ * same in binary32 (add, sub, mul, div, sqrt, acos; operands must be floats,
+ * results are floats widened exactly), op 14 the outward conversion of a to
+ * binary32.  Arithmetic parity checks (acos: ivarp/src/interval_acos.cpp:29-66
+ * rounds with MPFR). */
 int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi);
 
 /* Interval operations on the device (run-time sign decisions, as for operands

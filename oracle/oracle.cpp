@@ -11,6 +11,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <stdexcept>
@@ -40,7 +41,12 @@ thread_local std::uint64_t t_phase_units[kPhases];
 thread_local int t_phase = kPhaseOther;
 
 inline double opq(double x) { __asm__ volatile("" : "+x"(x)); return x; }
+inline float opq(float x) { __asm__ volatile("" : "+x"(x)); return x; }
 inline void count(int k) { if(g_count_flops) t_phase_flops[t_phase] += static_cast<std::uint64_t>(k); }
+// binary32 operations (the FP32 prefilter) are counted apart
+thread_local std::uint64_t t_phase_flops32[kPhases];
+static std::atomic<std::uint64_t> g_phase_flops32[kPhases];
+inline void count32(int k) { if(g_count_flops) t_phase_flops32[t_phase] += static_cast<std::uint64_t>(k); }
 inline void enter_phase(int mode, std::size_t level, std::uint64_t units) {
     t_phase = level < static_cast<std::size_t>(kPhaseLevels) ? mode * kPhaseLevels + static_cast<int>(level) : kPhaseOther;
     if(g_count_flops) t_phase_units[t_phase] += units;
@@ -51,32 +57,49 @@ void flush_thread_counts() {
         g_flops += t_phase_flops[i];
         g_phase_flops[i] += t_phase_flops[i];
         g_phase_units[i] += t_phase_units[i];
+        g_phase_flops32[i] += t_phase_flops32[i];
         t_phase_flops[i] = 0;
         t_phase_units[i] = 0;
+        t_phase_flops32[i] = 0;
     }
     t_phase = kPhaseOther;
 }
 
+// The whole oracle runs with the FPU in round-toward-minus-infinity (SSE for
+// binary64 and binary32 alike), so each operation is one correctly rounded
+// downward result; upper bounds use RU(x) = -RD(-x), square roots toggle the
+// mode (math_fn/interval_sqrt.hpp:72-116).
 inline double rd_add(double a, double b) { count(1); return opq(opq(a) + opq(b)); }
-inline double ru_add(double a, double b) { return -rd_add(-a, -b); }
 inline double rd_sub(double a, double b) { count(1); return opq(opq(a) - opq(b)); }
-inline double ru_sub(double a, double b) { return -rd_add(-a, b); }
 inline double rd_mul(double a, double b) { count(1); return opq(opq(a) * opq(b)); }
-inline double ru_mul(double a, double b) { return -rd_mul(-a, b); }
 inline double rd_div(double a, double b) { count(1); return opq(opq(a) / opq(b)); }
-inline double ru_div(double a, double b) { return -rd_div(-a, b); }
 inline double rd_sqrt(double a) { count(1); return opq(__builtin_sqrt(opq(a))); }
-inline double ru_sqrt(double a) {
-    count(1);
+inline float rd_add(float a, float b) { count32(1); return opq(opq(a) + opq(b)); }
+inline float rd_sub(float a, float b) { count32(1); return opq(opq(a) - opq(b)); }
+inline float rd_mul(float a, float b) { count32(1); return opq(opq(a) * opq(b)); }
+inline float rd_div(float a, float b) { count32(1); return opq(opq(a) / opq(b)); }
+inline float rd_sqrt(float a) { count32(1); return opq(__builtin_sqrtf(opq(a))); }
+template<class X> inline X ru_add(X a, X b) { return -rd_add(-a, -b); }
+template<class X> inline X ru_sub(X a, X b) { return -rd_add(-a, b); }
+template<class X> inline X ru_mul(X a, X b) { return -rd_mul(-a, b); }
+template<class X> inline X ru_div(X a, X b) { return -rd_div(-a, b); }
+inline double sqrt_up(double a) { return __builtin_sqrt(a); }
+inline float sqrt_up(float a) { return __builtin_sqrtf(a); }
+template<class X> inline X ru_sqrt(X a) {
+    if(sizeof(X) == 4) count32(1); else count(1);
     __asm__ volatile("" ::: "memory");
     std::fesetround(FE_UPWARD);
     __asm__ volatile("" ::: "memory");
-    double r = opq(__builtin_sqrt(opq(a)));
+    X r = opq(sqrt_up(opq(a)));
     __asm__ volatile("" ::: "memory");
     std::fesetround(FE_DOWNWARD);
     __asm__ volatile("" ::: "memory");
     return r;
 }
+// Outward conversion of a double to float (rounding down, resp. up).
+inline float f32_down(double d) { return static_cast<float>(opq(d)); }
+inline float f32_up(double d) { return -static_cast<float>(opq(-d)); }
+
 struct RoundDownGuard {
     int old;
     RoundDownGuard() : old(std::fegetround()) { std::fesetround(FE_DOWNWARD); }
@@ -84,30 +107,32 @@ struct RoundDownGuard {
 };
 
 constexpr double kInf = std::numeric_limits<double>::infinity();
+template<class X> constexpr X inf_v() { return std::numeric_limits<X>::infinity(); }
 constexpr std::int64_t kMaxB = INT64_C(9223372036854775807);
 constexpr std::int64_t kMinB = -kMaxB;
 
-struct IV { double lb, ub; bool und; };
+template<class X> struct IVT { X lb, ub; bool und; };
+using IV = IVT<double>;
 struct IB { bool def, poss; };
 
 // compile-time bound classification helpers (fixed_point_bounds.hpp:72-180)
 struct CT { std::int64_t lb, ub; };
 inline bool ct_finite(CT b) { return b.lb > kMinB && b.ub < kMaxB; }
-inline bool nonneg(CT b, const IV& f) { if(b.lb >= 0) return true; if(b.ub <= 0) return false; return f.lb >= 0; }
-inline bool nonpos(CT b, const IV& f) { if(b.ub <= 0) return true; if(b.lb >= 0) return false; return f.ub <= 0; }
-inline bool positive(CT b, const IV& f) { if(b.lb > 0) return true; if(b.ub <= 0) return false; return f.lb > 0; }
-inline bool negative(CT b, const IV& f) { if(b.ub < 0) return true; if(b.lb >= 0) return false; return f.ub < 0; }
+template<class X> inline bool nonneg(CT b, const IVT<X>& f) { if(b.lb >= 0) return true; if(b.ub <= 0) return false; return f.lb >= 0; }
+template<class X> inline bool nonpos(CT b, const IVT<X>& f) { if(b.ub <= 0) return true; if(b.lb >= 0) return false; return f.ub <= 0; }
+template<class X> inline bool positive(CT b, const IVT<X>& f) { if(b.lb > 0) return true; if(b.ub <= 0) return false; return f.lb > 0; }
+template<class X> inline bool negative(CT b, const IVT<X>& f) { if(b.ub < 0) return true; if(b.lb >= 0) return false; return f.ub < 0; }
 const CT kUnbounded{kMinB, kMaxB};
 
-inline double nan_to_zero(double v) { return v != v ? 0.0 : v; }
-inline void do_mul(double l1, double l2, double u1, double u2, bool finite, double& lo, double& hi) {
+template<class X> inline X nan_to_zero(X v) { return v != v ? X(0) : v; }
+template<class X> inline void do_mul(X l1, X l2, X u1, X u2, bool finite, X& lo, X& hi) {
     lo = rd_mul(l1, l2);
     hi = ru_mul(u1, u2);
     if(!finite) { lo = nan_to_zero(lo); hi = nan_to_zero(hi); }
 }
-IV mul_check_signs(const IV& a, const IV& b, CT B1, CT B2) {
+template<class X> IVT<X> mul_check_signs(const IVT<X>& a, const IVT<X>& b, CT B1, CT B2) {
     const bool fin = ct_finite(B1) && ct_finite(B2);
-    double lo, hi;
+    X lo, hi;
     if(nonneg(B1, a)) {
         if(nonneg(B2, b)) do_mul(a.lb, b.lb, a.ub, b.ub, fin, lo, hi);
         else if(nonpos(B2, b)) do_mul(a.ub, b.lb, a.lb, b.ub, fin, lo, hi);
@@ -121,44 +146,45 @@ IV mul_check_signs(const IV& a, const IV& b, CT B1, CT B2) {
     } else if(nonpos(B2, b)) {
         do_mul(a.ub, b.lb, a.lb, b.lb, fin, lo, hi);
     } else {
-        double l1, h1, l2, h2;
+        X l1, h1, l2, h2;
         do_mul(a.lb, b.ub, a.lb, b.lb, fin, l1, h1);
         do_mul(b.lb, a.ub, a.ub, b.ub, fin, l2, h2);
         lo = (l2 <= l1) ? l2 : l1;
         hi = (h1 <= h2) ? h2 : h1;
     }
-    return IV{lo, hi, a.und || b.und};
+    return IVT<X>{lo, hi, a.und || b.und};
 }
 
-inline int sgn(double v) { return (v > 0) - (v < 0); }
-void divide_by_zero(IV& num, const IV& den, int sl, int su, int dsum) {
+template<class X> inline int sgn(X v) { return (v > 0) - (v < 0); }
+template<class X> void divide_by_zero(IVT<X>& num, const IVT<X>& den, int sl, int su, int dsum) {
+    const X inf = inf_v<X>();
     num.und = true;
-    if(dsum == 0 || su - sl == 2) { num.lb = -kInf; num.ub = kInf; return; }
+    if(dsum == 0 || su - sl == 2) { num.lb = -inf; num.ub = inf; return; }
     if(dsum > 0) {
-        if(sl >= 0) { num.lb = rd_div(num.lb, den.ub); num.ub = kInf; }
-        else { num.ub = ru_div(num.ub, den.ub); num.lb = -kInf; }
+        if(sl >= 0) { num.lb = rd_div(num.lb, den.ub); num.ub = inf; }
+        else { num.ub = ru_div(num.ub, den.ub); num.lb = -inf; }
     } else {
-        if(sl >= 0) { num.ub = ru_div(num.lb, den.lb); num.lb = -kInf; }
-        else { num.lb = rd_div(num.ub, den.lb); num.ub = kInf; }
+        if(sl >= 0) { num.ub = ru_div(num.lb, den.lb); num.lb = -inf; }
+        else { num.lb = rd_div(num.ub, den.lb); num.ub = inf; }
     }
 }
-struct WInf { double v; int inf_sign; };
-inline double div_lb_inf(WInf n, WInf d) {
+template<class X> struct WInf { X v; int inf_sign; };
+template<class X> inline X div_lb_inf(WInf<X> n, WInf<X> d) {
     if(!n.inf_sign && !d.inf_sign) return rd_div(n.v, d.v);
-    if(d.inf_sign != 0) return 0.0;
-    return -kInf;
+    if(d.inf_sign != 0) return X(0);
+    return -inf_v<X>();
 }
-inline double div_ub_inf(WInf n, WInf d) {
+template<class X> inline X div_ub_inf(WInf<X> n, WInf<X> d) {
     if(!n.inf_sign && !d.inf_sign) return ru_div(n.v, d.v);
-    if(d.inf_sign != 0) return 0.0;
-    return kInf;
+    if(d.inf_sign != 0) return X(0);
+    return inf_v<X>();
 }
-void divide_by_infinite(IV& num, const IV& den) {
+template<class X> void divide_by_infinite(IVT<X>& num, const IVT<X>& den) {
     num.und = num.und || den.und;
     int snl = sgn(num.lb), snu = sgn(num.ub), sdl = sgn(den.lb), sdu = sgn(den.ub);
-    WInf nl{num.lb, std::isfinite(num.lb) ? 0 : -1}, nu{num.ub, std::isfinite(num.ub) ? 0 : 1};
-    WInf dl{den.lb, std::isfinite(den.lb) ? 0 : -1}, du{den.ub, std::isfinite(den.ub) ? 0 : 1};
-    auto set = [&](WInf a, WInf b, WInf c, WInf d) { double l = div_lb_inf(a, b), u = div_ub_inf(c, d); num.lb = l; num.ub = u; };
+    WInf<X> nl{num.lb, std::isfinite(num.lb) ? 0 : -1}, nu{num.ub, std::isfinite(num.ub) ? 0 : 1};
+    WInf<X> dl{den.lb, std::isfinite(den.lb) ? 0 : -1}, du{den.ub, std::isfinite(den.ub) ? 0 : 1};
+    auto set = [&](WInf<X> a, WInf<X> b, WInf<X> c, WInf<X> d) { X l = div_lb_inf(a, b), u = div_ub_inf(c, d); num.lb = l; num.ub = u; };
     if(sdl > 0) {
         if(snl >= 0) set(nl, du, nu, dl);
         else if(snu <= 0) set(nl, dl, nu, du);
@@ -171,12 +197,12 @@ void divide_by_infinite(IV& num, const IV& den) {
         divide_by_zero(num, den, snl, snu, sdl + sdu);
     }
 }
-IV interval_div(IV num, const IV& den, CT BN, CT BD) {
+template<class X> IVT<X> interval_div(IVT<X> num, const IVT<X>& den, CT BN, CT BD) {
     bool fin_ct = ct_finite(BN) && ct_finite(BD);
     bool fin_rt = std::isfinite(num.lb) && std::isfinite(num.ub) && std::isfinite(den.lb) && std::isfinite(den.ub);
     if(!(fin_ct || fin_rt)) { divide_by_infinite(num, den); return num; }
     num.und = num.und || den.und;
-    auto ia = [&](double a, double b, double c, double d) { double l = rd_div(a, b), u = ru_div(c, d); num.lb = l; num.ub = u; };
+    auto ia = [&](X a, X b, X c, X d) { X l = rd_div(a, b), u = ru_div(c, d); num.lb = l; num.ub = u; };
     if(positive(BD, den)) {
         if(nonneg(BN, num)) ia(num.lb, den.ub, num.ub, den.lb);
         else if(nonpos(BN, num)) ia(num.lb, den.lb, num.ub, den.ub);
@@ -190,15 +216,15 @@ IV interval_div(IV num, const IV& den, CT BN, CT BD) {
     }
     return num;
 }
-IV interval_sqrt(const IV& x, CT B) {
-    double lb = x.lb, ub = x.ub;
+template<class X> IVT<X> interval_sqrt(const IVT<X>& x, CT B) {
+    X lb = x.lb, ub = x.ub;
     bool und = x.und;
     if(!nonneg(B, x)) {
-        if(ub < 0) return IV{-kInf, kInf, true};
+        if(ub < 0) return IVT<X>{-inf_v<X>(), inf_v<X>(), true};
         lb = 0;
         und = true;
     }
-    return IV{rd_sqrt(lb), ru_sqrt(ub), und};
+    return IVT<X>{rd_sqrt(lb), ru_sqrt(ub), und};
 }
 // acos rounded down / up, correctly (MPFR at 53 bits), like the reference's
 // cpu_acos_rd / cpu_acos_ru (ivarp/src/interval_acos.cpp:29-66).  NaN outside [-1, 1].
@@ -220,11 +246,17 @@ IV interval_acos(IV x, CT B) {
     if(B.ub > kOne && x.ub > 1.0) { x.ub = 1.0; und = true; }
     return IV{acos_dir(x.ub, false), acos_dir(x.lb, true), und};
 }
+// binary32: the binary64 directed results converted outward (the largest float
+// below RD64(acos x) is the largest float below acos x)
+IVT<float> interval_acos(IVT<float> x, CT B) {
+    const IV r = interval_acos(IV{x.lb, x.ub, x.und}, B);
+    return IVT<float>{f32_down(r.lb), f32_up(r.ub), r.und};
+}
 
-inline double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }
-inline double dmin(double n1, double n2) { return n1 < n2 ? n1 : n2; }
+template<class X> inline X dmax(X n1, X n2) { return n2 < n1 ? n1 : n2; }
+template<class X> inline X dmin(X n1, X n2) { return n1 < n2 ? n1 : n2; }
 
-inline IB less(const IV& a, const IV& b) {
+template<class X> inline IB less(const IVT<X>& a, const IVT<X>& b) {
     return IB{!a.und && !b.und && a.ub < b.lb, a.und || b.und || a.lb < b.ub};
 }
 inline IB inot(IB x) { return IB{!x.poss, !x.def}; }
@@ -334,16 +366,27 @@ Program parse(const std::string& text) {
 }
 
 // ---------------------------------------------------------------- evaluation
-struct Cuboid {
-    std::vector<double> lb, ub;
+template<class X> struct CuboidT {
+    std::vector<X> lb, ub;
     std::uint64_t depth = 0;
 };
+using Cuboid = CuboidT<double>;
 
-struct Evaluator {
+// Interval evaluation of the program's nodes on one box, in binary64 or (the
+// FP32 prefilter) in binary32 with every constant converted outward.
+template<class X> struct EvaluatorT {
+    using IV = IVT<X>;
+    using Cuboid = CuboidT<X>;
     const Program& p;
     std::vector<IV> v;
     std::vector<IB> pb;
-    explicit Evaluator(const Program& prog) : p(prog), v(prog.nodes.size()), pb(prog.nodes.size()) {}
+    std::vector<X> c_lb, c_ub;
+    explicit EvaluatorT(const Program& prog) : p(prog), v(prog.nodes.size()), pb(prog.nodes.size()) {
+        for(const PNode& n : prog.nodes) {
+            if(sizeof(X) == 4) { c_lb.push_back(static_cast<X>(f32_down(n.c_lb))); c_ub.push_back(static_cast<X>(f32_up(n.c_ub))); }
+            else { c_lb.push_back(static_cast<X>(n.c_lb)); c_ub.push_back(static_cast<X>(n.c_ub)); }
+        }
+    }
 
     void eval_node(int id, const Cuboid& c) {
         const PNode& n = p.nodes[static_cast<std::size_t>(id)];
@@ -352,7 +395,7 @@ struct Evaluator {
         auto KCT = [&](int i) -> CT { return p.nodes[static_cast<std::size_t>(n.kids[static_cast<std::size_t>(i)])].ct; };
         IV& r = v[static_cast<std::size_t>(id)];
         switch(n.op) {
-            case C_CONST: r = IV{n.c_lb, n.c_ub, n.c_undef}; return;
+            case C_CONST: r = IV{c_lb[static_cast<std::size_t>(id)], c_ub[static_cast<std::size_t>(id)], n.c_undef}; return;
             case C_ARG: r = IV{c.lb[static_cast<std::size_t>(n.arg)], c.ub[static_cast<std::size_t>(n.arg)], false}; return;
             case C_NEG: r = IV{-K(0).ub, -K(0).lb, K(0).und}; return;
             case C_ADD: r = IV{rd_add(K(0).lb, K(1).lb), ru_add(K(0).ub, K(1).ub), K(0).und || K(1).und}; return;
@@ -475,14 +518,18 @@ struct Evaluator {
     }
 };
 
+using Evaluator = EvaluatorT<double>;
+using Evaluator32 = EvaluatorT<float>;
+
 enum BoundID { B_NONE = 0, B_LB = 1, B_UB = 2, B_BOTH = 3, B_EMPTY = 4 };
 
-int apply_bound(Evaluator& ev, int k, Cuboid& c) {
+template<class X>
+int apply_bound(EvaluatorT<X>& ev, int k, CuboidT<X>& c) {
     const PBound& b = ev.p.bounds[static_cast<std::size_t>(k)];
-    IV iv = ev.eval_bound(k, c);
+    IVT<X> iv = ev.eval_bound(k, c);
     if(b.honor && iv.und) return B_NONE;
-    double& tl = c.lb[static_cast<std::size_t>(b.target)];
-    double& tu = c.ub[static_cast<std::size_t>(b.target)];
+    X& tl = c.lb[static_cast<std::size_t>(b.target)];
+    X& tu = c.ub[static_cast<std::size_t>(b.target)];
     if(b.dir == 1) {
         if(iv.ub < tu) { tu = iv.ub; return iv.ub < tl ? B_EMPTY : B_UB; }
     } else if(b.dir == 2) {
@@ -522,7 +569,8 @@ struct IndexQueue {
     }
 };
 
-bool propagate(Evaluator& ev, Cuboid& c, int arg, std::size_t it_limit, unsigned rec_limit) {
+template<class X>
+bool propagate(EvaluatorT<X>& ev, CuboidT<X>& c, int arg, std::size_t it_limit, unsigned rec_limit) {
     const Program& p = ev.p;
     const std::size_t nb = p.bounds.size();
     IndexQueue q(nb);
@@ -576,7 +624,12 @@ struct Splitter {
 
 struct Counts {
     std::uint64_t cuboids = 0, leaves = 0, criticals = 0, repeated = 0;
-    Counts& operator+=(const Counts& o) { cuboids += o.cuboids; leaves += o.leaves; criticals += o.criticals; repeated += o.repeated; return *this; }
+    std::uint64_t f32_checked = 0, f32_decided = 0;   // FP32 prefilter (static-level children)
+    Counts& operator+=(const Counts& o) {
+        cuboids += o.cuboids; leaves += o.leaves; criticals += o.criticals; repeated += o.repeated;
+        f32_checked += o.f32_checked; f32_decided += o.f32_decided;
+        return *this;
+    }
 };
 
 struct Core {
@@ -593,14 +646,26 @@ struct Core {
         for(int cn : p.rows[static_cast<std::size_t>(p.N - 1)]) if(!ev.possibly(cn, c)) return false;
         return true;
     }
-    bool violates_rows(Evaluator& ev, const Cuboid& c, int from, int to) const {
+    template<class X>
+    bool violates_rows(EvaluatorT<X>& ev, const CuboidT<X>& c, int from, int to) const {
         for(int r = from; r < to; ++r)
             for(int cn : p.rows[static_cast<std::size_t>(r)]) if(!ev.possibly(cn, c)) return true;
         return false;
     }
-    // Static enumeration (cpu_prover_core.hpp:221-256); `leaf` is called for each surviving leaf.
+    // FP32 prefilter of one static-level child: true when binary32 interval
+    // arithmetic on the outward-converted box proves it empty or infeasible.
+    bool f32_prunes(Evaluator32& ev32, const Cuboid& q, int arg, int row_end) const {
+        CuboidT<float> f;
+        for(std::size_t a = 0; a < q.lb.size(); ++a) { f.lb.push_back(f32_down(q.lb[a])); f.ub.push_back(f32_up(q.ub[a])); }
+        if(propagate(ev32, f, arg, static_cast<std::size_t>(p.N), 4)) return true;
+        return violates_rows(ev32, f, arg, row_end);
+    }
+    // Static enumeration (cpu_prover_core.hpp:221-256); `leaf` is called for each
+    // surviving leaf.  With ev32 set, each child is first tried in binary32
+    // (a child pruned there is a leaf; the others are decided in binary64).
     template<typename Leaf>
-    void enumerate(Evaluator& ev, const Cuboid& e, std::size_t level, int mode, Counts& cnt, Leaf&& leaf) const {
+    void enumerate(Evaluator& ev, const Cuboid& e, std::size_t level, int mode, Counts& cnt, Leaf&& leaf,
+                   Evaluator32* ev32 = nullptr) const {
         if(level == static_splits.size()) { cnt.leaves++; leaf(e); return; }
         const PSplit& s = p.splits[static_cast<std::size_t>(static_splits[level])];
         const int arg = s.arg;
@@ -614,14 +679,18 @@ struct Core {
             Cuboid q = e;
             q.lb[static_cast<std::size_t>(arg)] = sp.point(i);
             q.ub[static_cast<std::size_t>(arg)] = sp.point(i + 1);
+            if(ev32) {
+                cnt.f32_checked++;
+                if(f32_prunes(*ev32, q, arg, row_end)) { cnt.f32_decided++; cnt.leaves++; continue; }
+            }
             if(propagate(ev, q, arg, static_cast<std::size_t>(p.N), 4)) { cnt.leaves++; continue; }
             if(violates_rows(ev, q, arg, row_end)) { cnt.leaves++; continue; }
-            enumerate(ev, q, level + 1, mode, cnt, leaf);
+            enumerate(ev, q, level + 1, mode, cnt, leaf, ev32);
         }
         cnt.cuboids += static_cast<std::uint64_t>(subs);
     }
     // Non-final pass with max_iterations_per_node = 1: the hull of surviving leaves.
-    bool nonfinal(Evaluator& ev, const Cuboid& e, Cuboid& hull, Counts& cnt) const {
+    bool nonfinal(Evaluator& ev, const Cuboid& e, Cuboid& hull, Counts& cnt, Evaluator32* ev32 = nullptr) const {
         bool empty = true;
         enumerate(ev, e, 0, 0, cnt, [&](const Cuboid& leaf) {
             if(empty) { hull = leaf; empty = false; return; }
@@ -630,13 +699,13 @@ struct Core {
                 if(leaf.lb[i] < hull.lb[i]) hull.lb[i] = leaf.lb[i];
                 if(leaf.ub[i] > hull.ub[i]) hull.ub[i] = leaf.ub[i];
             }
-        });
+        }, ev32);
         if(!empty) hull.depth = e.depth;
         return !empty;
     }
     template<typename Crit>
-    void final_pass(Evaluator& ev, const Cuboid& e, Counts& cnt, Crit&& crit) const {
-        enumerate(ev, e, 0, 1, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); });
+    void final_pass(Evaluator& ev, const Cuboid& e, Counts& cnt, Crit&& crit, Evaluator32* ev32 = nullptr) const {
+        enumerate(ev, e, 0, 1, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); }, ev32);
     }
     // Dynamic split (pd_impl_split_into.hpp:29-75) or initial split (proof_driver.hpp:320-391).
     template<typename Out>
@@ -713,7 +782,8 @@ struct RunResult {
     std::vector<Cuboid> criticals;
 };
 
-void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations, int threads, RunResult& res) {
+void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations, int threads, RunResult& res,
+                    bool fp32 = false) {
     Core core(p);
     Frontier f;
     f.stack = std::move(start);
@@ -724,15 +794,17 @@ void run_from_queue(const Program& p, std::vector<Cuboid> start, int generations
         ts.emplace_back([&, t] {
             RoundDownGuard g;
             Evaluator ev(p);
+            Evaluator32 ev32(p);
+            Evaluator32* e32 = fp32 ? &ev32 : nullptr;
             Counts& cnt = counts[static_cast<std::size_t>(t)];
             std::vector<Cuboid> batch, produced;
             while(f.pop(batch, 32)) {
                 for(const Cuboid& e : batch) {
                     Cuboid hull;
-                    if(!core.nonfinal(ev, e, hull, cnt)) continue;
+                    if(!core.nonfinal(ev, e, hull, cnt, e32)) continue;
                     if(hull.depth >= static_cast<std::uint64_t>(generations)) {
                         std::vector<Cuboid> crit;
-                        core.final_pass(ev, hull, cnt, [&](const Cuboid& c) { crit.push_back(c); });
+                        core.final_pass(ev, hull, cnt, [&](const Cuboid& c) { crit.push_back(c); }, e32);
                         if(!crit.empty()) {
                             std::lock_guard<std::mutex> l(res_m);
                             for(auto& c : crit) res.criticals.push_back(c);
@@ -756,6 +828,8 @@ thread_local std::string t_error;
 
 struct Handle {
     Program prog;
+    bool fp32 = false;       // FP32 prefilter in the static levels
+    Counts last;             // counts of the last computing call
 };
 
 template<typename F> int guarded(F&& f) {
@@ -787,7 +861,7 @@ Cuboid soa_get(const Program& p, std::uint64_t n, const double* lb, const double
 extern "C" {
 
 int oracle_load(const char* text, void** handle) {
-    return guarded([&] { auto* h = new Handle{parse(text)}; *handle = h; });
+    return guarded([&] { auto h = std::make_unique<Handle>(); h->prog = parse(text); *handle = h.release(); });
 }
 void oracle_free(void* handle) { delete static_cast<Handle*>(handle); }
 int oracle_nargs(void* handle) { return static_cast<Handle*>(handle)->prog.N; }
@@ -827,8 +901,10 @@ int oracle_run(void* handle, int generations, int threads, OracleCounts* counts,
         std::vector<Cuboid> q = initial_queue(p, init);
         // the reference pops the last element first; reverse-free order does not change counts
         RunResult r;
-        run_from_queue(p, std::move(q), generations, threads < 1 ? 1 : threads, r);
+        Handle* h = static_cast<Handle*>(handle);
+        run_from_queue(p, std::move(q), generations, threads < 1 ? 1 : threads, r, h->fp32);
         r.counts += init;
+        h->last = r.counts;
         to_out(r.counts, counts);
         if(n_criticals) *n_criticals = r.criticals.size();
         for(std::uint64_t i = 0; criticals && i < r.criticals.size() && i < max_criticals; ++i)
@@ -851,7 +927,9 @@ int oracle_run_slice(void* handle, int generations, int threads, std::uint64_t f
         if(first > last) first = last;
         std::vector<Cuboid> part(q.begin() + static_cast<std::ptrdiff_t>(first), q.begin() + static_cast<std::ptrdiff_t>(last));
         RunResult r;
-        run_from_queue(p, std::move(part), generations, threads < 1 ? 1 : threads, r);
+        Handle* h = static_cast<Handle*>(handle);
+        run_from_queue(p, std::move(part), generations, threads < 1 ? 1 : threads, r, h->fp32);
+        h->last = r.counts;
         to_out(r.counts, counts);
     });
 }
@@ -863,16 +941,20 @@ int oracle_nonfinal(void* handle, std::uint64_t n, const double* lb, const doubl
         const Program& p = static_cast<Handle*>(handle)->prog;
         Core core(p);
         Evaluator ev(p);
+        Evaluator32 ev32(p);
+        Handle* hd = static_cast<Handle*>(handle);
+        Evaluator32* e32 = hd->fp32 ? &ev32 : nullptr;
         Counts cnt;
         for(std::uint64_t i = 0; i < n; ++i) {
             Cuboid e = soa_get(p, n, lb, ub, i), h;
-            bool ok = core.nonfinal(ev, e, h, cnt);
+            bool ok = core.nonfinal(ev, e, h, cnt, e32);
             has_hull[i] = ok ? 1 : 0;
             for(int a = 0; a < p.N; ++a) {
                 hull_lb[static_cast<std::uint64_t>(a) * n + i] = ok ? h.lb[static_cast<std::size_t>(a)] : 0.0;
                 hull_ub[static_cast<std::uint64_t>(a) * n + i] = ok ? h.ub[static_cast<std::size_t>(a)] : 0.0;
             }
         }
+        hd->last = cnt;
         to_out(cnt, counts);
     });
 }
@@ -884,6 +966,9 @@ int oracle_final(void* handle, std::uint64_t n, const double* lb, const double* 
         const Program& p = static_cast<Handle*>(handle)->prog;
         Core core(p);
         Evaluator ev(p);
+        Evaluator32 ev32(p);
+        Handle* hd = static_cast<Handle*>(handle);
+        Evaluator32* e32 = hd->fp32 ? &ev32 : nullptr;
         Counts cnt;
         std::uint64_t k = 0;
         for(std::uint64_t i = 0; i < n; ++i) {
@@ -895,9 +980,10 @@ int oracle_final(void* handle, std::uint64_t n, const double* lb, const double* 
                         crit_ub[k * static_cast<std::uint64_t>(p.N) + static_cast<std::uint64_t>(a)] = c.ub[static_cast<std::size_t>(a)];
                     }
                 ++k;
-            });
+            }, e32);
         }
         *n_crit = k;
+        hd->last = cnt;
         to_out(cnt, counts);
     });
 }
@@ -938,6 +1024,30 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
                 case 3: lo[i] = rd_div(a[i], b[i]); hi[i] = ru_div(a[i], b[i]); break;
                 case 4: lo[i] = rd_sqrt(a[i]); hi[i] = ru_sqrt(a[i]); break;
                 case 5: lo[i] = acos_dir(a[i], false); hi[i] = acos_dir(a[i], true); break;
+                case 8: case 9: case 10: case 11: case 12: {
+                    const float x = static_cast<float>(a[i]), y = static_cast<float>(b[i]);
+                    if(static_cast<double>(x) != a[i] || static_cast<double>(y) != b[i])
+                        if(!(a[i] != a[i]) && !(b[i] != b[i])) throw std::invalid_argument("binary32 operands must be floats");
+                    float l = 0, h = 0;
+                    switch(op) {
+                        case 8: l = rd_add(x, y); h = ru_add(x, y); break;
+                        case 9: l = rd_sub(x, y); h = ru_sub(x, y); break;
+                        case 10: l = rd_mul(x, y); h = ru_mul(x, y); break;
+                        case 11: l = rd_div(x, y); h = ru_div(x, y); break;
+                        default: l = rd_sqrt(x); h = ru_sqrt(x); break;
+                    }
+                    lo[i] = l;
+                    hi[i] = h;
+                    break;
+                }
+                case 13: {
+                    const float x = static_cast<float>(a[i]);
+                    const IVT<float> r = interval_acos(IVT<float>{x, x, false}, kUnbounded);
+                    lo[i] = r.lb;
+                    hi[i] = r.ub;
+                    break;
+                }
+                case 14: lo[i] = f32_down(a[i]); hi[i] = f32_up(a[i]); break;
                 default: throw std::invalid_argument("unknown op");
             }
         }
@@ -980,7 +1090,7 @@ int oracle_interval_op(int op, std::uint64_t n, const double* a, const double* b
 std::uint64_t oracle_flop_count(void) { return g_flops.load(); }
 void oracle_flop_reset(void) {
     g_flops = 0;
-    for(int i = 0; i < kPhases; ++i) { g_phase_flops[i] = 0; g_phase_units[i] = 0; }
+    for(int i = 0; i < kPhases; ++i) { g_phase_flops[i] = 0; g_phase_units[i] = 0; g_phase_flops32[i] = 0; }
 }
 int oracle_phase_counts(int mode, int level, std::uint64_t* flops, std::uint64_t* units) {
     if(mode < 0 || mode > 4 || level < 0 || level >= kPhaseLevels || !flops || !units) return -1;
@@ -990,5 +1100,23 @@ int oracle_phase_counts(int mode, int level, std::uint64_t* flops, std::uint64_t
     return 0;
 }
 void oracle_flop_enable(int on) { g_count_flops = on != 0; }
+int oracle_phase_counts32(int mode, int level, std::uint64_t* flops) {
+    if(mode < 0 || mode > 4 || level < 0 || level >= kPhaseLevels || !flops) return -1;
+    const int ph = mode == 4 ? kPhaseOther : mode * kPhaseLevels + level;
+    *flops = g_phase_flops32[ph].load();
+    return 0;
+}
+
+int oracle_set_fp32_prefilter(void* handle, int on) {
+    if(!handle) return -1;
+    static_cast<Handle*>(handle)->fp32 = on != 0;
+    return 0;
+}
+int oracle_fp32_stats(void* handle, std::uint64_t* checked, std::uint64_t* decided) {
+    if(!handle || !checked || !decided) return -1;
+    *checked = static_cast<Handle*>(handle)->last.f32_checked;
+    *decided = static_cast<Handle*>(handle)->last.f32_decided;
+    return 0;
+}
 
 } // extern "C"

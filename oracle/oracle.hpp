@@ -21,6 +21,8 @@
 //   hull and criticals
 //   driver                 prover/proof_driver.hpp:300-438,472-509, prover/pd_impl_split_into.hpp:29-75
 //   initial box            refactor_constraint_system/refactor_constraints_impl.hpp:61-101
+//   binary32 intervals     the same operations on float (ivarp's IFloat number type,
+//                          number/interval.hpp:346-347), for the FP32 prefilter
 #pragma once
 #include <cstdint>
 #include <cstddef>
@@ -69,6 +71,8 @@ int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* 
 
 // Elementwise directed-rounding primitives for arithmetic parity tests.
 // op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 acos(a).  out_lo = RD(a op b), out_hi = RU(a op b).
+// Ops 8..13: the same in binary32 (operands must be floats; results widened
+// exactly), 14: outward conversion of a to binary32.
 int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* out_lo, double* out_hi);
 
 // Interval operations on n operand pairs, 3 doubles per interval (lb, ub,
@@ -86,4 +90,15 @@ void oracle_flop_enable(int on);
 // processed: mode 0 non-final static level, 1 final static level, 2 dynamic split
 // level, 3 initial-queue level, 4 everything else (level ignored).
 int oracle_phase_counts(int mode, int level, std::uint64_t* flops, std::uint64_t* units);
+// Binary32 operations of the phase (FP32 prefilter).
+int oracle_phase_counts32(int mode, int level, std::uint64_t* flops);
+
+// FP32 prefilter (no counterpart in the reference): with it on, every child of a
+// static level is first propagated and row-checked in outward-rounded binary32;
+// a child binary32 proves empty or infeasible is a pruned leaf, the others are
+// decided in binary64 as usual.  Applies to oracle_run, oracle_run_slice,
+// oracle_nonfinal and oracle_final.
+int oracle_set_fp32_prefilter(void* handle, int on);
+// Static-level children of the last computing call that binary32 evaluated / pruned.
+int oracle_fp32_stats(void* handle, std::uint64_t* checked, std::uint64_t* decided);
 }

@@ -63,6 +63,9 @@ def lib():
             "oracle_flop_enable": (None, [ctypes.c_int]),
             "oracle_interval_op": (ctypes.c_int, [ctypes.c_int, _U64, _D, _D, _D]),
             "oracle_phase_counts": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+            "oracle_phase_counts32": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64)]),
+            "oracle_set_fp32_prefilter": (ctypes.c_int, [_P, ctypes.c_int]),
+            "oracle_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -82,10 +85,18 @@ def _dp(a):
 
 
 class Oracle:
-    def __init__(self, program_text):
+    def __init__(self, program_text, fp32=False):
         self._h = _P()
         _check(lib().oracle_load(program_text.encode(), ctypes.byref(self._h)))
         self.N = lib().oracle_nargs(self._h)
+        if fp32:
+            _check(lib().oracle_set_fp32_prefilter(self._h, 1))
+
+    def fp32_stats(self):
+        """(checked, decided) static-level children of the last call under the FP32 prefilter."""
+        a, b = _U64(), _U64()
+        _check(lib().oracle_fp32_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def close(self):
         if self._h:
@@ -167,7 +178,8 @@ class Oracle:
 
 
 def rounded_op(op, a, b):
-    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5}[op]
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5, "add32": 8, "sub32": 9, "mul32": 10,
+            "div32": 11, "sqrt32": 12, "acos32": 13, "to32": 14}[op]
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     lo = np.zeros_like(a)
@@ -189,13 +201,15 @@ def interval_op(op, a, b=None):
     return out
 
 
-def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=None, timeout=None):
+def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=None, timeout=None, fp32=False):
     """Run the oracle CLI on a program file; returns the parsed JSON line."""
     import json
     ensure_built()
     cmd = [str(CLI), str(program_path), "--threads", str(threads), "--generations", str(generations)]
     if flops:
         cmd.append("--flops")
+    if fp32:
+        cmd.append("--fp32")
     if slice_range is not None:
         cmd += ["--slice", str(slice_range[0]), str(slice_range[1])]
     r = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=timeout)

@@ -21,7 +21,27 @@ __global__ void k_rounded_op(int op, unsigned long long n, const double* a, cons
             case 3: l = rd_div(x, y); h = ru_div(x, y); break;
             case 4: l = rd_sqrt(x); h = ru_sqrt(x); break;
             case 5: { bool hard; acosfx::acos_directed(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
-            default: break;
+            case 13: {   // binary32 acos: outward conversion of the binary64 directed results
+                const IVT<float> r = iv_acos<false, false>(IVT<float>{static_cast<float>(x), static_cast<float>(x), false});
+                l = r.lb; h = r.ub;
+                break;
+            }
+            case 14: l = f32_down(x); h = f32_up(x); break;
+            default: {
+                if(op < 8 || op > 12) break;
+                const float fx = static_cast<float>(x), fy = static_cast<float>(y);   // exact: operands are floats
+                float fl = 0, fh = 0;
+                switch(op) {
+                    case 8: fl = rd_add(fx, fy); fh = ru_add(fx, fy); break;
+                    case 9: fl = rd_sub(fx, fy); fh = ru_sub(fx, fy); break;
+                    case 10: fl = rd_mul(fx, fy); fh = ru_mul(fx, fy); break;
+                    case 11: fl = rd_div(fx, fy); fh = ru_div(fx, fy); break;
+                    default: fl = rd_sqrt(fx); fh = ru_sqrt(fx); break;
+                }
+                l = fl;
+                h = fh;
+                break;
+            }
         }
         lo[i] = l;
         hi[i] = h;
@@ -82,7 +102,7 @@ extern "C" int sqd_interval_op(int device, int op, size_t n, const double* a, co
 }
 
 extern "C" int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi) {
-    if(op < 0 || op > 5 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
+    if(op < 0 || (op > 5 && op < 8) || op > 14 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
     if(n == 0) return SQD_OK;
     double *da = nullptr, *db = nullptr, *dl = nullptr, *dh = nullptr;
     const size_t bytes = n * sizeof(double);

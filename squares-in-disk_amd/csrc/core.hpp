@@ -46,6 +46,8 @@ constexpr int kBlock = 256;
 #endif
 
 enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
+// Flag or'ed into a static-level mode at launch: the FP32-prefiltered kernel.
+constexpr int kModeF32 = 4;
 
 struct DeviceCounters {
     unsigned long long cuboids;
@@ -79,6 +81,10 @@ struct LevelArgs {
     unsigned long long* block_counts;    // per-block count slots (block_counts_add)
     const double* consts;
     unsigned generations;
+    // FP32 prefilter: children (static-level child indices) binary32 left undecided
+    unsigned long long* undec;
+    unsigned long long* undec_count;
+    unsigned long long undec_cap;
 };
 
 // ---------------------------------------------------------------- hull keys
@@ -117,9 +123,9 @@ __device__ __forceinline__ void rec_inc(unsigned long long (&rec)[3], int a) {
     if(a < 16) rec[0] += inc; else if(a < 32) rec[1] += inc; else rec[2] += inc;
 }
 
-template<class P>
-__device__ bool propagate(double (&lb)[P::N], double (&ub)[P::N], int arg, int it_limit, unsigned rec_limit,
-                          const double* C, PropScratch<P>& s)
+template<class P, class Num>
+__device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int arg, int it_limit, unsigned rec_limit,
+                          const Num* C, PropScratch<P>& s)
 {
     constexpr int NB = P::NB;
     static_assert(NB <= 48, "at most 48 bounds (4-bit recursion counters in three words)");
@@ -187,34 +193,45 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
-// Adds the block's cuboid, leaf and critical counts to the block's own slot of
-// `slots` (4 words per block index).  Device-scope atomics on one address from
+// Words of one per-block count slot: cuboids, leaves, criticals, children
+// decided by the FP32 prefilter, children it checked.
+constexpr int kSlotWords = 8;
+
+// Adds the block's counts to the block's own slot of `slots` (kSlotWords words
+// per block index).  Device-scope atomics on one address from
 // all XCDs serialise, so the counts are not accumulated in one place on the
 // device: blocks of one launch own distinct slots and launches on a stream do
 // not overlap, so a plain read-modify-write is exact, and the host sums the
 // slots.  Every thread of the block calls this once, at the end of the kernel.
 __device__ inline void block_counts_add(unsigned long long* slots, unsigned long long cub, unsigned long long leaf,
-                                        unsigned long long crit) {
-    __shared__ unsigned long long part[3][kBlock / 64];
-    cub = wave_sum(cub);
-    leaf = wave_sum(leaf);
-    crit = wave_sum(crit);
+                                        unsigned long long crit, unsigned long long f32_dec = 0,
+                                        unsigned long long f32_chk = 0) {
+    __shared__ unsigned long long part[5][kBlock / 64];
+    unsigned long long v[5] = {cub, leaf, crit, f32_dec, f32_chk};
     const unsigned w = threadIdx.x / 64u;
-    if(__lane_id() == 0) { part[0][w] = cub; part[1][w] = leaf; part[2][w] = crit; }
+#pragma unroll
+    for(int j = 0; j < 5; ++j) {
+        v[j] = wave_sum(v[j]);
+        if(__lane_id() == 0) part[j][w] = v[j];
+    }
     __syncthreads();
-    if(threadIdx.x == 0) {
-        unsigned long long a = 0, b = 0, c = 0;
-        for(unsigned i = 0; i < blockDim.x / 64u; ++i) { a += part[0][i]; b += part[1][i]; c += part[2][i]; }
-        unsigned long long* slot = slots + 4ull * blockIdx.x;
-        if(a) slot[0] += a;
-        if(b) slot[1] += b;
-        if(c) slot[2] += c;
+    if(threadIdx.x < 5) {
+        unsigned long long a = 0;
+        for(unsigned i = 0; i < blockDim.x / 64u; ++i) a += part[threadIdx.x][i];
+        unsigned long long* slot = slots + static_cast<unsigned long long>(kSlotWords) * blockIdx.x;
+        if(a) slot[threadIdx.x] += a;
     }
 }
 
 template<class P>
 __device__ __forceinline__ void load_consts(double* dst, const double* src) {
     for(int i = threadIdx.x; i < 2 * P::NC; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+// Binary32 constants: each binary64 constant interval converted outward.
+template<class P>
+__device__ __forceinline__ void load_consts_f32(float* dst, const double* src) {
+    for(int i = threadIdx.x; i < 2 * P::NC; i += blockDim.x) dst[i] = (i & 1) ? f32_up(src[i]) : f32_down(src[i]);
     __syncthreads();
 }
 
@@ -300,71 +317,156 @@ __device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, u
 }
 
 // ---------------------------------------------------------------- static levels
-// Level L of the static grid (modes NONFINAL and FINAL).
-template<class P, int MODE, int L>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs args) {
-    constexpr int N = P::N;
+// Child w of a static level: the parent box with the level's variable replaced
+// by subrange w % SUBS of the splitter (splitter.hpp:44-59), its dynamic cuboid
+// (origin) and its enumeration index within that cuboid's grid (leaf).
+template<class P, int L>
+__device__ __forceinline__ void static_child(const LevelArgs& args, unsigned long long in_base, unsigned long long w,
+                                             double (&lb)[P::N], double (&ub)[P::N], unsigned& origin, unsigned& leaf) {
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
+    const unsigned long long item = w / SUBS;
+    const int i = static_cast<int>(w % SUBS);
+    const unsigned long long src = in_base + item;
+    load_box<P>(args.in, src, lb, ub);
+    if(args.in.origin) { origin = args.in.origin[src]; leaf = args.in.leaf[src]; }
+    else { origin = static_cast<unsigned>(item); leaf = 0; }
+    leaf = leaf * SUBS + static_cast<unsigned>(i);
+    const double pl = lb[ARG], pu = ub[ARG];
+    const double wd = split_width(pl, pu, SUBS);
+    lb[ARG] = split_point(pl, pu, wd, i, SUBS);
+    ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
+}
+
+// The binary64 decision for child w (cpu_prover_core.hpp:221-256): propagate,
+// check the level's constraint rows, then count the leaf, join the hull (last
+// non-final level), record a critical (last final level) or append the survivor
+// to the next level's list.  Called by every lane of the wave (the hull join and
+// the append are wave-wide); `valid` marks the lanes that hold a child.
+template<class P, int MODE, int L>
+__device__ __forceinline__ void static_decide(const LevelArgs& args, unsigned long long in_base, bool valid,
+                                              unsigned long long w, const double* s_c, PropScratch<P>& s_prop,
+                                              unsigned long long& c_leaf, unsigned long long& c_crit) {
+    constexpr int N = P::N;
+    constexpr int ARG = P::st_arg(L);
     constexpr bool LAST = (L == P::NSTAT - 1);
+    bool survive = false, join = false;
+    double lb[N], ub[N];
+    unsigned origin = 0, leaf = 0;
+    if(valid) {
+        static_child<P, L>(args, in_base, w, lb, ub, origin, leaf);
+        if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+            ++c_leaf;
+        } else if(P::template rows_violated<L>(lb, ub, s_c)) {
+            ++c_leaf;
+        } else if(LAST) {
+            ++c_leaf;
+            if(MODE == FINAL) { ++c_crit; survive = true; }
+            else join = true;
+        } else {
+            survive = true;
+        }
+    }
+    if(LAST && MODE == NONFINAL) hull_join<N>(args.hull, join, origin, leaf, lb, ub);
+    if(!LAST || MODE == FINAL) {
+        const unsigned long long pos = wave_append(survive, args.out.count);
+        if(survive) {
+            if(pos < args.out.cap) {
+                store_box<P>(args.out, pos, lb, ub);
+                if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
+            } else {
+                atomicAdd(&args.counters->overflow, 1ull);
+            }
+        }
+    }
+}
+
+// Level L of the static grid (modes NONFINAL and FINAL).
+//
+// F32 = false: every child w of the level's parents is decided in binary64.
+// F32 = true: the children listed by k_static_prefilter (those binary32 left
+// undecided) are decided in binary64, exactly as above.
+template<class P, int MODE, int L, bool F32>
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs args) {
+    constexpr int SUBS = P::st_subs(L);
+    const unsigned long long n_items = *args.in_count;
+    const unsigned long long total = F32 ? *args.undec_count : n_items * static_cast<unsigned long long>(SUBS);
+    const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(F32 && static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // block-uniform
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
     __shared__ PropScratch<P> s_prop;
     load_consts<P>(s_c, args.consts);
+    enter_round_down();
+    const unsigned long long in_base = input_base(args);
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
+    // every lane of a wave runs the same number of loop trips, so the wave-wide
+    // ballots always see all lanes of the wave
+    const unsigned long long trips = (total + stride - 1) / stride;
+    for(unsigned long long trip = 0; trip < trips; ++trip) {
+        const unsigned long long t = first + trip * stride;
+        const bool valid = t < total;
+        unsigned long long w = t;
+        if(F32) w = valid ? args.undec[t] : 0ull;
+        else if(valid && w % SUBS == 0) c_cub += SUBS;
+        static_decide<P, MODE, L>(args, in_base, valid, w, s_c, s_prop, c_leaf, c_crit);
+    }
+    block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
+}
+
+// The FP32 prefilter of static level L: every child is propagated and checked
+// in binary32 on its outward-rounded box with outward binary32 constants.  A
+// child that binary32 proves empty or infeasible is a pruned leaf (a rigorous
+// verdict: the binary32 intervals enclose the real ranges); the others are
+// appended to args.undec for k_static_level<..., true>, which decides them in
+// binary64.  Survivors, hulls and criticals therefore always come from the
+// binary64 evaluation; only the set of pruned leaves may differ from the plain
+// pass.  The binary32 pass runs in its own kernel so that its register budget
+// (and with it the occupancy) is not that of the binary64 code.
+template<class P, int MODE, int L>
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(LevelArgs args) {
+    constexpr int N = P::N;
+    constexpr int ARG = P::st_arg(L);
+    constexpr int SUBS = P::st_subs(L);
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ float s_c32[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ PropScratch<P> s_prop;
+    load_consts<P>(s_c, args.consts);
+    load_consts_f32<P>(s_c32, s_c);
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
     const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-    unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    // every lane of a wave runs the same number of loop trips, so the wave-wide
-    // ballots below always see all lanes of the wave
     const unsigned long long trips = (total + stride - 1) / stride;
+    unsigned long long c_cub = 0, c_leaf = 0, c_dec = 0, c_chk = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
         const unsigned long long w = first + trip * stride;
-        const bool valid = w < total;
-        bool survive = false, join = false;
-        double lb[N], ub[N];
-        unsigned origin = 0, leaf = 0;
-        if(valid) {
-            const unsigned long long item = w / SUBS;
-            const int i = static_cast<int>(w % SUBS);
-            const unsigned long long src = in_base + item;
-            load_box<P>(args.in, src, lb, ub);
-            if(args.in.origin) { origin = args.in.origin[src]; leaf = args.in.leaf[src]; }
-            else { origin = static_cast<unsigned>(item); leaf = 0; }
-            leaf = leaf * SUBS + static_cast<unsigned>(i);
-            if(i == 0) c_cub += SUBS;
-            const double pl = lb[ARG], pu = ub[ARG];
-            const double wd = split_width(pl, pu, SUBS);
-            lb[ARG] = split_point(pl, pu, wd, i, SUBS);
-            ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
-            if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+        bool undecided = false;
+        if(w < total) {
+            if(w % SUBS == 0) c_cub += SUBS;
+            double lb[N], ub[N];
+            unsigned origin, leaf;
+            static_child<P, L>(args, in_base, w, lb, ub, origin, leaf);
+            float lb32[N], ub32[N];
+#pragma unroll
+            for(int a = 0; a < N; ++a) { lb32[a] = f32_down(lb[a]); ub32[a] = f32_up(ub[a]); }
+            ++c_chk;
+            if(propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop) || P::template rows_violated<L>(lb32, ub32, s_c32)) {
                 ++c_leaf;
-            } else if(P::template rows_violated<L>(lb, ub, s_c)) {
-                ++c_leaf;
-            } else if(LAST) {
-                ++c_leaf;
-                if(MODE == FINAL) { ++c_crit; survive = true; }
-                else join = true;
+                ++c_dec;
             } else {
-                survive = true;
+                undecided = true;
             }
         }
-        if(LAST && MODE == NONFINAL) hull_join<N>(args.hull, join, origin, leaf, lb, ub);
-        if(!LAST || MODE == FINAL) {
-            const unsigned long long pos = wave_append(survive, args.out.count);
-            if(survive) {
-                if(pos < args.out.cap) {
-                    store_box<P>(args.out, pos, lb, ub);
-                    if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
-                } else {
-                    atomicAdd(&args.counters->overflow, 1ull);
-                }
-            }
+        const unsigned long long pos = wave_append(undecided, args.undec_count);
+        if(undecided) {
+            if(pos < args.undec_cap) args.undec[pos] = w;
+            else atomicAdd(&args.counters->overflow, 1ull);
         }
     }
-    block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
+    block_counts_add(args.block_counts, c_cub, c_leaf, 0ull, c_dec, c_chk);
 }
 
 // ---------------------------------------------------------------- dynamic levels
@@ -568,6 +670,56 @@ __global__ void k_initial_box(const double* box_lb, const double* box_ub, const 
         store_box<P>(list, 0, lb, ub);
         if(list.depth) list.depth[0] = 0;
         *list.count = 1;
+    }
+}
+
+// FP32-versus-FP64 width of each constraint's compared difference (left minus
+// right side of a top-level comparison) over n boxes given in SoA form (stride n).
+// Per constraint j: out[3j] += sum of width32 / width64, out[3j + 1] += number of
+// boxes where both widths are finite, defined and width64 > 0, out[3j + 2] = the
+// largest ratio (bit pattern of a positive double, maximised as an integer).
+// Statistics only: the quotient is not directionally rounded.
+template<class P>
+__global__ __launch_bounds__(kBlock) void k_width_stats(const double* lb, const double* ub, unsigned long long n,
+                                                        const double* consts, double* out) {
+    constexpr int N = P::N;
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ float s_c32[2 * (P::NC > 0 ? P::NC : 1)];
+    load_consts<P>(s_c, consts);
+    load_consts_f32<P>(s_c32, s_c);
+    enter_round_down();
+    const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    double bl[N], bu[N];
+    float fl[N], fu[N];
+#pragma unroll
+    for(int a = 0; a < N; ++a) {
+        bl[a] = valid ? lb[static_cast<unsigned long long>(a) * n + i] : 0.0;
+        bu[a] = valid ? ub[static_cast<unsigned long long>(a) * n + i] : 0.0;
+        fl[a] = f32_down(bl[a]);
+        fu[a] = f32_up(bu[a]);
+    }
+    for(int j = 0; j < P::NCONS; ++j) {
+        double r = 0.0;
+        bool ok = false;
+        if(valid) {
+            const IV d = P::cons_diff(j, bl, bu, s_c);
+            const IVT<float> f = P::cons_diff(j, fl, fu, s_c32);
+            const double w64 = d.ub - d.lb;
+            const double w32 = static_cast<double>(f.ub) - static_cast<double>(f.lb);
+            ok = !d.und && !f.und && is_finite(w64) && is_finite(w32) && w64 > 0.0;
+            r = ok ? w32 / w64 : 0.0;
+        }
+        double s = r;
+#pragma unroll
+        for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        const unsigned long long cnt = wave_sum(ok ? 1ull : 0ull);
+        const double mx = wave_max(ok ? r : 0.0);
+        if(__lane_id() == 0 && cnt) {
+            atomicAdd(out + 3 * j, s);
+            atomicAdd(out + 3 * j + 1, static_cast<double>(cnt));
+            atomicMax(reinterpret_cast<unsigned long long*>(out + 3 * j + 2), bits(mx));
+        }
     }
 }
 

@@ -1,4 +1,5 @@
-// Interval operations of the GPU prover core (binary64, outward rounded).
+// Interval operations of the GPU prover core (outward rounded; binary64, and
+// binary32 for the FP32 prefilter: every operation is a template over the bound type).
 //
 // Semantics follow the reference's IDouble operations bit for bit:
 //   + -            ivarp/include/ivarp/number/float_interval_addsub.hpp:53-102
@@ -19,19 +20,22 @@
 namespace sqd {
 namespace dev {
 
-struct IV { double lb, ub; bool und; };
+template<class R> struct IVT { R lb, ub; bool und; };
+using IV = IVT<double>;
 struct IB { bool def, poss; };
 
 enum Tri : int { F = 0, T = 1, R = 2 };
 
-__device__ __forceinline__ IV iv_add(IV a, IV b) { return IV{rd_add(a.lb, b.lb), ru_add(a.ub, b.ub), a.und || b.und}; }
-__device__ __forceinline__ IV iv_sub(IV a, IV b) { return IV{rd_sub(a.lb, b.ub), ru_sub(a.ub, b.lb), a.und || b.und}; }
-__device__ __forceinline__ IV iv_neg(IV a) { return IV{-a.ub, -a.lb, a.und}; }
+template<class X> __device__ __forceinline__ X inf_v() { return static_cast<X>(__builtin_huge_val()); }
 
-__device__ __forceinline__ double nan_to_zero(double v) { return v != v ? 0.0 : v; }
+template<class X> __device__ __forceinline__ IVT<X> iv_add(IVT<X> a, IVT<X> b) { return IVT<X>{rd_add(a.lb, b.lb), ru_add(a.ub, b.ub), a.und || b.und}; }
+template<class X> __device__ __forceinline__ IVT<X> iv_sub(IVT<X> a, IVT<X> b) { return IVT<X>{rd_sub(a.lb, b.ub), ru_sub(a.ub, b.lb), a.und || b.und}; }
+template<class X> __device__ __forceinline__ IVT<X> iv_neg(IVT<X> a) { return IVT<X>{-a.ub, -a.lb, a.und}; }
 
-template<bool Finite>
-__device__ __forceinline__ void do_mul(double l1, double l2, double u1, double u2, double& lo, double& hi) {
+template<class X> __device__ __forceinline__ X nan_to_zero(X v) { return v != v ? X(0) : v; }
+
+template<bool Finite, class X>
+__device__ __forceinline__ void do_mul(X l1, X l2, X u1, X u2, X& lo, X& hi) {
     lo = rd_mul(l1, l2);
     hi = ru_mul(u1, u2);
     if(!Finite) { lo = nan_to_zero(lo); hi = nan_to_zero(hi); }
@@ -41,11 +45,11 @@ template<int C> __device__ __forceinline__ bool tri(bool runtime) { return C == 
 
 // Multiplication with the reference's nine sign cases.  NN/NP: "non-negative" /
 // "non-positive" decisions for each operand.
-template<int NN1, int NP1, int NN2, int NP2, bool Finite>
-__device__ inline IV iv_mul(IV a, IV b) {
-    double lo, hi;
-    const bool a_nn = tri<NN1>(a.lb >= 0.0), a_np = tri<NP1>(a.ub <= 0.0);
-    const bool b_nn = tri<NN2>(b.lb >= 0.0), b_np = tri<NP2>(b.ub <= 0.0);
+template<int NN1, int NP1, int NN2, int NP2, bool Finite, class X>
+__device__ inline IVT<X> iv_mul(IVT<X> a, IVT<X> b) {
+    X lo, hi;
+    const bool a_nn = tri<NN1>(a.lb >= X(0)), a_np = tri<NP1>(a.ub <= X(0));
+    const bool b_nn = tri<NN2>(b.lb >= X(0)), b_np = tri<NP2>(b.ub <= X(0));
     if(a_nn) {
         if(b_nn) do_mul<Finite>(a.lb, b.lb, a.ub, b.ub, lo, hi);
         else if(b_np) do_mul<Finite>(a.ub, b.lb, a.lb, b.ub, lo, hi);
@@ -59,26 +63,28 @@ __device__ inline IV iv_mul(IV a, IV b) {
     } else if(b_np) {
         do_mul<Finite>(a.ub, b.lb, a.lb, b.lb, lo, hi);
     } else {
-        double l1, h1, l2, h2;
+        X l1, h1, l2, h2;
         do_mul<Finite>(a.lb, b.ub, a.lb, b.lb, l1, h1);
         do_mul<Finite>(b.lb, a.ub, a.ub, b.ub, l2, h2);
         lo = (l2 <= l1) ? l2 : l1;
         hi = (h1 <= h2) ? h2 : h1;
     }
-    return IV{lo, hi, a.und || b.und};
+    return IVT<X>{lo, hi, a.und || b.und};
 }
 
 // square(x) = x * x with run-time sign checks, then bounded below by 0.
-__device__ inline IV iv_square(IV a) {
-    IV r = iv_mul<R, R, R, R, false>(a, a);
-    if(r.lb < 0.0) r.lb = 0.0;
+template<class X>
+__device__ inline IVT<X> iv_square(IVT<X> a) {
+    IVT<X> r = iv_mul<R, R, R, R, false>(a, a);
+    if(r.lb < X(0)) r.lb = X(0);
     return r;
 }
 
-__device__ __forceinline__ int sgn(double v) { return (v > 0.0) - (v < 0.0); }
+template<class X> __device__ __forceinline__ int sgn(X v) { return (v > X(0)) - (v < X(0)); }
 
-__device__ inline void divide_by_zero(IV& num, IV den, int sl, int su, int dsum) {
-    const double inf = __builtin_huge_val();
+template<class X>
+__device__ inline void divide_by_zero(IVT<X>& num, IVT<X> den, int sl, int su, int dsum) {
+    const X inf = inf_v<X>();
     num.und = true;
     if(dsum == 0 || su - sl == 2) { num.lb = -inf; num.ub = inf; return; }
     if(dsum > 0) {
@@ -90,23 +96,26 @@ __device__ inline void divide_by_zero(IV& num, IV den, int sl, int su, int dsum)
     }
 }
 
-__device__ __forceinline__ double div_lb_inf(double n, int ni, double d, int di) {
+template<class X>
+__device__ __forceinline__ X div_lb_inf(X n, int ni, X d, int di) {
     if(!ni && !di) return rd_div(n, d);
-    if(di != 0) return 0.0;
-    return -__builtin_huge_val();
+    if(di != 0) return X(0);
+    return -inf_v<X>();
 }
-__device__ __forceinline__ double div_ub_inf(double n, int ni, double d, int di) {
+template<class X>
+__device__ __forceinline__ X div_ub_inf(X n, int ni, X d, int di) {
     if(!ni && !di) return ru_div(n, d);
-    if(di != 0) return 0.0;
-    return __builtin_huge_val();
+    if(di != 0) return X(0);
+    return inf_v<X>();
 }
 
-__device__ inline void divide_by_infinite(IV& num, IV den) {
+template<class X>
+__device__ inline void divide_by_infinite(IVT<X>& num, IVT<X> den) {
     num.und = num.und || den.und;
     const int snl = sgn(num.lb), snu = sgn(num.ub), sdl = sgn(den.lb), sdu = sgn(den.ub);
     const int inl = is_finite(num.lb) ? 0 : -1, inu = is_finite(num.ub) ? 0 : 1;
     const int idl = is_finite(den.lb) ? 0 : -1, idu = is_finite(den.ub) ? 0 : 1;
-    double l, u;
+    X l, u;
     if(sdl > 0) {
         if(snl >= 0) { l = div_lb_inf(num.lb, inl, den.ub, idu); u = div_ub_inf(num.ub, inu, den.lb, idl); }
         else if(snu <= 0) { l = div_lb_inf(num.lb, inl, den.lb, idl); u = div_ub_inf(num.ub, inu, den.ub, idu); }
@@ -125,21 +134,21 @@ __device__ inline void divide_by_infinite(IV& num, IV den) {
 
 // Division; POS/NEG: "denominator positive/negative"; NN/NP for the numerator;
 // CTFinite: both operands have finite compile-time bounds.
-template<int NN, int NP, int POS, int NEG, bool CTFinite>
-__device__ inline IV iv_div(IV num, IV den) {
+template<int NN, int NP, int POS, int NEG, bool CTFinite, class X>
+__device__ inline IVT<X> iv_div(IVT<X> num, IVT<X> den) {
     if(!CTFinite && !(is_finite(num.lb) && is_finite(num.ub) && is_finite(den.lb) && is_finite(den.ub))) {
         divide_by_infinite(num, den);
         return num;
     }
     num.und = num.und || den.und;
-    double l, u;
-    if(tri<POS>(den.lb > 0.0)) {
-        if(tri<NN>(num.lb >= 0.0)) { l = rd_div(num.lb, den.ub); u = ru_div(num.ub, den.lb); }
-        else if(tri<NP>(num.ub <= 0.0)) { l = rd_div(num.lb, den.lb); u = ru_div(num.ub, den.ub); }
+    X l, u;
+    if(tri<POS>(den.lb > X(0))) {
+        if(tri<NN>(num.lb >= X(0))) { l = rd_div(num.lb, den.ub); u = ru_div(num.ub, den.lb); }
+        else if(tri<NP>(num.ub <= X(0))) { l = rd_div(num.lb, den.lb); u = ru_div(num.ub, den.ub); }
         else { l = rd_div(num.lb, den.lb); u = ru_div(num.ub, den.lb); }
-    } else if(tri<NEG>(den.ub < 0.0)) {
-        if(tri<NN>(num.lb >= 0.0)) { l = rd_div(num.ub, den.ub); u = ru_div(num.lb, den.lb); }
-        else if(tri<NP>(num.ub <= 0.0)) { l = rd_div(num.ub, den.lb); u = ru_div(num.lb, den.ub); }
+    } else if(tri<NEG>(den.ub < X(0))) {
+        if(tri<NN>(num.lb >= X(0))) { l = rd_div(num.ub, den.ub); u = ru_div(num.lb, den.lb); }
+        else if(tri<NP>(num.ub <= X(0))) { l = rd_div(num.ub, den.lb); u = ru_div(num.lb, den.ub); }
         else { l = rd_div(num.ub, den.ub); u = ru_div(num.lb, den.ub); }
     } else {
         divide_by_zero(num, den, sgn(num.lb), sgn(num.ub), sgn(den.lb) + sgn(den.ub));
@@ -150,16 +159,16 @@ __device__ inline IV iv_div(IV num, IV den) {
     return num;
 }
 
-template<int NN>
-__device__ inline IV iv_sqrt(IV x) {
-    double lb = x.lb, ub = x.ub;
+template<int NN, class X>
+__device__ inline IVT<X> iv_sqrt(IVT<X> x) {
+    X lb = x.lb, ub = x.ub;
     bool und = x.und;
-    if(!tri<NN>(x.lb >= 0.0)) {
-        if(ub < 0.0) return IV{-__builtin_huge_val(), __builtin_huge_val(), true};
-        lb = 0.0;
+    if(!tri<NN>(x.lb >= X(0))) {
+        if(ub < X(0)) return IVT<X>{-inf_v<X>(), inf_v<X>(), true};
+        lb = X(0);
         und = true;
     }
-    return IV{rd_sqrt(lb), ru_sqrt(ub), und};
+    return IVT<X>{rd_sqrt(lb), ru_sqrt(ub), und};
 }
 
 // Evaluations of acos whose rounding direction could not be decided (see
@@ -181,35 +190,44 @@ __device__ inline IV iv_acos(IV x) {
     if(hard1 || hard2) atomicAdd(&g_acos_hard, 1ull);
     return IV{l1, h2, und};
 }
+// The binary32 version: the binary64 directed results converted outward (the
+// largest float below RD64(acos x) is the largest float below acos x).
+template<bool CheckLb, bool CheckUb>
+__device__ inline IVT<float> iv_acos(IVT<float> x) {
+    const IV r = iv_acos<CheckLb, CheckUb>(IV{static_cast<double>(x.lb), static_cast<double>(x.ub), x.und});
+    return IVT<float>{f32_down(r.lb), f32_up(r.ub), r.und};
+}
 
-__device__ __forceinline__ double dmax(double n1, double n2) { return n2 < n1 ? n1 : n2; }
-__device__ __forceinline__ double dmin(double n1, double n2) { return n1 < n2 ? n1 : n2; }
-__device__ __forceinline__ IV iv_max(IV a, IV b) { return IV{dmax(a.lb, b.lb), dmax(a.ub, b.ub), a.und || b.und}; }
-__device__ __forceinline__ IV iv_min(IV a, IV b) { return IV{dmin(a.lb, b.lb), dmin(a.ub, b.ub), a.und || b.und}; }
+template<class X> __device__ __forceinline__ X dmax(X n1, X n2) { return n2 < n1 ? n1 : n2; }
+template<class X> __device__ __forceinline__ X dmin(X n1, X n2) { return n1 < n2 ? n1 : n2; }
+template<class X> __device__ __forceinline__ IVT<X> iv_max(IVT<X> a, IVT<X> b) { return IVT<X>{dmax(a.lb, b.lb), dmax(a.ub, b.ub), a.und || b.und}; }
+template<class X> __device__ __forceinline__ IVT<X> iv_min(IVT<X> a, IVT<X> b) { return IVT<X>{dmin(a.lb, b.lb), dmin(a.ub, b.ub), a.und || b.und}; }
 
-__device__ __forceinline__ IV iv_ite(IB c, IV t, IV e) {
+template<class X>
+__device__ __forceinline__ IVT<X> iv_ite(IB c, IVT<X> t, IVT<X> e) {
     if(c.def) return t;
     if(!c.poss) return e;
-    IV r = t;
+    IVT<X> r = t;
     if(e.lb < r.lb) r.lb = e.lb;
     if(e.ub > r.ub) r.ub = e.ub;
     r.und = r.und || e.und;
     return r;
 }
 
-__device__ __forceinline__ IB ib_less(IV a, IV b) {
+template<class X>
+__device__ __forceinline__ IB ib_less(IVT<X> a, IVT<X> b) {
     return IB{!a.und && !b.und && a.ub < b.lb, a.und || b.und || a.lb < b.ub};
 }
 __device__ __forceinline__ IB ib_not(IB x) { return IB{!x.poss, !x.def}; }
-__device__ __forceinline__ IB ib_lt(IV a, IV b) { return ib_less(a, b); }
-__device__ __forceinline__ IB ib_gt(IV a, IV b) { return ib_less(b, a); }
-__device__ __forceinline__ IB ib_ge(IV a, IV b) { return ib_not(ib_less(a, b)); }
-__device__ __forceinline__ IB ib_le(IV a, IV b) { return ib_not(ib_less(b, a)); }
-__device__ __forceinline__ IB ib_eq(IV a, IV b) {
+template<class X> __device__ __forceinline__ IB ib_lt(IVT<X> a, IVT<X> b) { return ib_less(a, b); }
+template<class X> __device__ __forceinline__ IB ib_gt(IVT<X> a, IVT<X> b) { return ib_less(b, a); }
+template<class X> __device__ __forceinline__ IB ib_ge(IVT<X> a, IVT<X> b) { return ib_not(ib_less(a, b)); }
+template<class X> __device__ __forceinline__ IB ib_le(IVT<X> a, IVT<X> b) { return ib_not(ib_less(b, a)); }
+template<class X> __device__ __forceinline__ IB ib_eq(IVT<X> a, IVT<X> b) {
     return IB{!a.und && !b.und && a.lb == a.ub && b.lb == b.ub && a.lb == b.lb,
               a.und || b.und || (!(a.ub < b.lb) && !(b.ub < a.lb))};
 }
-__device__ __forceinline__ IB ib_neq(IV a, IV b) { return ib_not(ib_eq(a, b)); }
+template<class X> __device__ __forceinline__ IB ib_neq(IVT<X> a, IVT<X> b) { return ib_not(ib_eq(a, b)); }
 __device__ __forceinline__ IB ib_or(IB a, IB b) { return IB{a.def || b.def, a.poss || b.poss}; }
 __device__ __forceinline__ IB ib_and(IB a, IB b) { return IB{a.def && b.def, a.poss && b.poss}; }
 

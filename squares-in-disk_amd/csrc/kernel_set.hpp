@@ -18,7 +18,8 @@ struct KernelSet {
     // Level launches.  `max_children` bounds the number of children of the
     // launch (host-known upper bound); `cus` is the device's compute-unit count.
     // The grid is the kernel's resident capacity (occupancy x CUs), capped by the
-    // blocks the bound needs; a nonzero `grid` overrides it.
+    // blocks the bound needs; a nonzero `grid` overrides it.  A static-level mode
+    // or'ed with dev::kModeF32 selects the FP32-prefiltered variant.
     hipError_t (*static_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
                                unsigned cus, unsigned grid, hipStream_t s);
     hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
@@ -36,6 +37,12 @@ struct KernelSet {
                               dev::BoxList out, unsigned char* has, hipStream_t s);
     // Undecided acos roundings of this program so far (device-wide counter).
     hipError_t (*acos_hard)(unsigned long long* out);
+    // Constraints (roots of the constraint rows) and the FP32-versus-FP64 width
+    // statistics of their compared differences over n boxes (k_width_stats).
+    int NCONS;
+    int (*cons_id)(int j);
+    hipError_t (*width_stats)(const double* lb, const double* ub, unsigned long long n, const double* consts,
+                              double* out, hipStream_t s);
 };
 
 void register_kernel_set(const KernelSet* ks);
@@ -56,11 +63,20 @@ inline unsigned level_grid(unsigned resident, unsigned long long max_children, u
     return static_cast<unsigned>(need < 1 ? 1 : (need < resident ? need : resident));
 }
 
-template<class P, int MODE, int L>
+// One static level: the binary64 kernel over every child, or (F32) the binary32
+// prefilter followed by the binary64 kernel over the children it left undecided.
+template<class P, int MODE, int L, bool F32>
 hipError_t launch_one_static(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
                              hipStream_t s) {
-    static const unsigned per_cu = blocks_per_cu(dev::k_static_level<P, MODE, L>);
-    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L>), dim3(level_grid(per_cu * cus, max_children, grid)),
+    if(F32) {
+        static const unsigned pre_cu = blocks_per_cu(dev::k_static_prefilter<P, MODE, L>);
+        hipLaunchKernelGGL((dev::k_static_prefilter<P, MODE, L>), dim3(level_grid(pre_cu * cus, max_children, grid)),
+                           dim3(dev::kBlock), 0, s, a);
+        const hipError_t e = hipGetLastError();
+        if(e != hipSuccess) return e;
+    }
+    static const unsigned per_cu = blocks_per_cu(dev::k_static_level<P, MODE, L, F32>);
+    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L, F32>), dim3(level_grid(per_cu * cus, max_children, grid)),
                        dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
@@ -72,12 +88,12 @@ hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned long long max_ch
                        dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
-template<class P, int MODE, int... L>
+template<class P, int MODE, bool F32, int... L>
 hipError_t launch_static(int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus, unsigned grid,
                          hipStream_t s, std::integer_sequence<int, L...>) {
     hipError_t e = hipErrorInvalidValue;
     (void)a; (void)mc; (void)cus; (void)grid; (void)s; (void)level;
-    ((level == L ? (e = launch_one_static<P, MODE, L>(a, mc, cus, grid, s), 0) : 0), ...);
+    ((level == L ? (e = launch_one_static<P, MODE, L, F32>(a, mc, cus, grid, s), 0) : 0), ...);
     return e;
 }
 template<class P, int MODE, int... J>
@@ -93,8 +109,11 @@ template<class P>
 hipError_t static_level(int mode, int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus,
                         unsigned grid, hipStream_t s) {
     auto seq = std::make_integer_sequence<int, P::NSTAT>{};
-    if(mode == dev::NONFINAL) return launch_static<P, dev::NONFINAL>(level, a, mc, cus, grid, s, seq);
-    if(mode == dev::FINAL) return launch_static<P, dev::FINAL>(level, a, mc, cus, grid, s, seq);
+    if(mode == dev::NONFINAL) return launch_static<P, dev::NONFINAL, false>(level, a, mc, cus, grid, s, seq);
+    if(mode == dev::FINAL) return launch_static<P, dev::FINAL, false>(level, a, mc, cus, grid, s, seq);
+    if(mode == (dev::NONFINAL | dev::kModeF32))
+        return launch_static<P, dev::NONFINAL, true>(level, a, mc, cus, grid, s, seq);
+    if(mode == (dev::FINAL | dev::kModeF32)) return launch_static<P, dev::FINAL, true>(level, a, mc, cus, grid, s, seq);
     return hipErrorInvalidValue;
 }
 template<class P>
@@ -138,6 +157,17 @@ hipError_t hull_decode(dev::BoxList items, unsigned long long n, unsigned long l
     return hipGetLastError();
 }
 
+template<class P>
+int cons_id(int j) { return P::cons_id(j); }
+template<class P>
+hipError_t width_stats(const double* lb, const double* ub, unsigned long long n, const double* consts, double* out,
+                       hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    const unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);
+    hipLaunchKernelGGL((dev::k_width_stats<P>), dim3(grid), dim3(dev::kBlock), 0, s, lb, ub, n, consts, out);
+    return hipGetLastError();
+}
+
 inline hipError_t acos_hard(unsigned long long* out) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(dev::g_acos_hard), sizeof(unsigned long long), 0, hipMemcpyDeviceToHost);
 }
@@ -162,6 +192,9 @@ KernelSet make_kernel_set(std::uint64_t fingerprint, const char* key, const doub
     k.initial_box = &detail::initial_box<P>;
     k.hull_decode = &detail::hull_decode<P>;
     k.acos_hard = &detail::acos_hard;
+    k.NCONS = P::NCONS;
+    k.cons_id = &detail::cons_id<P>;
+    k.width_stats = &detail::width_stats<P>;
     return k;
 }
 

@@ -207,5 +207,107 @@ __device__ inline double ru_sqrt(double a) {
     return tiny ? __builtin_ldexp(s, -500) : s;
 }
 
+// ---------------------------------------------------------------- binary32
+// The same operations on floats (the FP32 prefilter).  The MODE write above sets
+// round-down for f32 too; the kernels keep f32 denormals (IEEE mode), so
+// directed results near zero are exact.
+__device__ __forceinline__ float rd_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float ru_add(float a, float b) { return -((-a) - b); }
+__device__ __forceinline__ float rd_sub(float a, float b) { return a - b; }
+__device__ __forceinline__ float ru_sub(float a, float b) { return -(b - a); }
+__device__ __forceinline__ float rd_mul(float a, float b) { return __builtin_fmaf(a, b, 0.0f); }
+__device__ __forceinline__ float ru_mul(float a, float b) { return -__builtin_fmaf(-a, b, 0.0f); }
+__device__ __forceinline__ float fms_rd(float a, float b, float c) { return __builtin_fmaf(-b, c, a); }
+
+__device__ __forceinline__ std::uint32_t bits(float x) { return __builtin_bit_cast(std::uint32_t, x); }
+__device__ __forceinline__ float from_bits32(std::uint32_t u) { return __builtin_bit_cast(float, u); }
+__device__ __forceinline__ bool is_finite(float x) { return (bits(x) & 0x7f800000u) != 0x7f800000u; }
+__device__ __forceinline__ float next_down(float x) {
+    const std::uint32_t u = bits(x);
+    if((u << 1) == 0) return from_bits32(0x80000001u);
+    if(u == 0x7f800000u) return from_bits32(0x7f7fffffu);     // +inf -> largest float
+    return from_bits32((u >> 31) ? u + 1 : u - 1);
+}
+__device__ __forceinline__ float next_up(float x) {
+    const std::uint32_t u = bits(x);
+    if((u << 1) == 0) return from_bits32(0x00000001u);
+    if(u == 0xff800000u) return from_bits32(0xff7fffffu);     // -inf -> most negative float
+    return from_bits32((u >> 31) ? u - 1 : u + 1);
+}
+
+// Outward conversion of a double: the largest float <= d / the smallest float
+// >= d (NaN stays NaN).  The hardware conversion rounds in the current mode;
+// the comparison with the exact widening makes the result independent of it.
+__device__ __forceinline__ float f32_down(double d) {
+    float f = static_cast<float>(d);
+    if(static_cast<double>(f) > d) f = next_down(f);
+    return f;
+}
+__device__ __forceinline__ float f32_up(double d) {
+    float f = static_cast<float>(d);
+    if(static_cast<double>(f) < d) f = next_up(f);
+    return f;
+}
+
+// Division: a float quotient rounded down equals the binary64 quotient rounded
+// down and then to float (every float is a double, so the largest float below
+// the exact quotient is also the largest float below its binary64 round-down).
+// Divisions are rare in the programs; this reuses the binary64 algorithm.
+__device__ inline float rd_div(float a, float b) {
+    return f32_down(rd_div(static_cast<double>(a), static_cast<double>(b)));
+}
+__device__ inline float ru_div(float a, float b) {
+    return f32_up(ru_div(static_cast<double>(a), static_cast<double>(b)));
+}
+
+// Square root of a float in [2^-100, 2^128): the hardware estimate
+// (v_sqrt_f32, a few ulp at worst in any rounding mode) corrected by the sign of
+// the exact residual m - s*s.  For m >= 2^-100 the root is >= 2^-50 and a
+// nonzero residual is at least 2^-96 in magnitude, so it never rounds to zero.
+__device__ __noinline__ float rd_sqrt_walk(float m, float s) {
+    for(int i = 0; i < 64; ++i) {
+        if(fms_rd(m, s, s) < 0.0f) s = next_down(s); else break;
+    }
+    for(int i = 0; i < 64; ++i) {
+        const float t = next_up(s);
+        if(fms_rd(m, t, t) >= 0.0f) s = t; else break;
+    }
+    return s;
+}
+__device__ __forceinline__ float rd_sqrt_fix(float m, float g) {
+    const float r0 = fms_rd(m, g, g);
+    const bool above = r0 < 0.0f;
+    const float s1 = from_bits32(bits(g) + (above ? ~0u : 1u));
+    const float r1 = fms_rd(m, s1, s1);
+    const bool ok = above ? (r1 >= 0.0f) : (r1 <= 0.0f);
+    const float res = above ? s1 : (r1 == 0.0f ? s1 : g);
+    if(__builtin_expect(!ok, 0)) return rd_sqrt_walk(m, g);
+    return res;
+}
+__device__ __forceinline__ float special_sqrt(float a) {
+    __asm__ volatile("" : "+v"(a));
+    return __builtin_sqrtf(a);
+}
+// Positive finite inputs below 2^-100 are scaled by 2^100 (the root by 2^50).
+__device__ __forceinline__ float scale_for_sqrt(float a, bool& tiny) {
+    tiny = a < 0x1p-100f;
+    return tiny ? a * 0x1p100f : a;   // exact: a power of two, no overflow
+}
+__device__ inline float rd_sqrt(float a) {
+    if(!(a > 0.0f) || !is_finite(a)) return special_sqrt(a);
+    bool tiny;
+    const float m = scale_for_sqrt(a, tiny);
+    const float s = rd_sqrt_fix(m, __builtin_amdgcn_sqrtf(m));
+    return tiny ? __builtin_ldexpf(s, -50) : s;
+}
+__device__ inline float ru_sqrt(float a) {
+    if(!(a > 0.0f) || !is_finite(a)) return special_sqrt(a);
+    bool tiny;
+    const float m = scale_for_sqrt(a, tiny);
+    float s = rd_sqrt_fix(m, __builtin_amdgcn_sqrtf(m));
+    if(fms_rd(m, s, s) != 0.0f) s = from_bits32(bits(s) + 1u);
+    return tiny ? __builtin_ldexpf(s, -50) : s;
+}
+
 } // namespace dev
 } // namespace sqd

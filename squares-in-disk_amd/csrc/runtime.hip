@@ -68,6 +68,9 @@ struct DeviceState {
     unsigned long long host_n;                // input count of host-driven calls
     unsigned long long frontier_full;         // nonzero: a round could not pop anything
     unsigned long long level_count[64];       // output counter of (mode, level): mode * 16 + level
+    unsigned long long f32_decided;           // static-level children pruned by the FP32 prefilter
+    unsigned long long f32_checked;           // static-level children the FP32 prefilter evaluated
+    unsigned long long undec_count[32];       // FP32 prefilter: undecided children of (mode, level), mode * 16 + level
 };
 
 static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a block of 64-bit words");
@@ -103,39 +106,37 @@ __global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigne
         st->spl_count = 0;
     }
     if(t < 64) st->level_count[t] = 0;
+    if(t < 32) st->undec_count[t] = 0;
 }
 // Adds the per-block count slots into the state's counters and clears them
 // (one block; run before the state is read).
 __global__ void k_fold_counts(DeviceState* st, unsigned long long* slots, unsigned long long n_slots) {
-    __shared__ unsigned long long part[3][256];
-    unsigned long long a = 0, b = 0, c = 0;
+    constexpr int W = 5;   // used words of a slot
+    __shared__ unsigned long long part[W][256];
+    unsigned long long acc[W] = {0, 0, 0, 0, 0};
     for(unsigned long long i = threadIdx.x; i < n_slots; i += blockDim.x) {
-        a += slots[4 * i + 0];
-        b += slots[4 * i + 1];
-        c += slots[4 * i + 2];
-        slots[4 * i + 0] = slots[4 * i + 1] = slots[4 * i + 2] = 0ull;
+        unsigned long long* s = slots + static_cast<unsigned long long>(dev::kSlotWords) * i;
+        for(int j = 0; j < W; ++j) { acc[j] += s[j]; s[j] = 0ull; }
     }
-    part[0][threadIdx.x] = a;
-    part[1][threadIdx.x] = b;
-    part[2][threadIdx.x] = c;
+    for(int j = 0; j < W; ++j) part[j][threadIdx.x] = acc[j];
     __syncthreads();
     for(unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
-        if(threadIdx.x < w) {
-            part[0][threadIdx.x] += part[0][threadIdx.x + w];
-            part[1][threadIdx.x] += part[1][threadIdx.x + w];
-            part[2][threadIdx.x] += part[2][threadIdx.x + w];
-        }
+        if(threadIdx.x < w)
+            for(int j = 0; j < W; ++j) part[j][threadIdx.x] += part[j][threadIdx.x + w];
         __syncthreads();
     }
     if(threadIdx.x == 0) {
         st->counters.cuboids += part[0][0];
         st->counters.leaves += part[1][0];
         st->counters.criticals += part[2][0];
+        st->f32_decided += part[3][0];
+        st->f32_checked += part[4][0];
     }
 }
 // Level counters to zero (between chunks of a host-driven call).
 __global__ void k_levels_reset(DeviceState* st) {
     if(blockIdx.x == 0 && threadIdx.x < 64) st->level_count[threadIdx.x] = 0;
+    if(blockIdx.x == 0 && threadIdx.x < 32) st->undec_count[threadIdx.x] = 0;
 }
 // hull record initial values: min-key, max-key, zero-sign lb, zero-sign ub
 __global__ void k_init_hull(unsigned long long* p, unsigned long long n_records) {
@@ -216,7 +217,7 @@ struct sqd_core {
     double* d_consts = nullptr;
     double* d_box_ct = nullptr;     // 2N: outward compile-time box
     double* d_box_init = nullptr;   // 2N: initial runtime box
-    sqd::DeviceState* d_state = nullptr;   // followed by n_slots count slots of 4 words
+    sqd::DeviceState* d_state = nullptr;   // followed by n_slots count slots of dev::kSlotWords words
     sqd::DeviceState* h_state = nullptr;   // pinned host copy of the same block
     unsigned long long* d_slots = nullptr;
     unsigned long long n_slots = 0;        // >= the largest grid of a counting kernel
@@ -225,6 +226,9 @@ struct sqd_core {
     bool dirty = true;                // hull records and count slots need a full reset before use
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
+    bool fp32 = false;               // static levels run the FP32-prefiltered kernels
+    unsigned long long* d_undec = nullptr;   // FP32 prefilter: undecided child indices of one level
+    unsigned long long undec_cap = 0;
     unsigned long long k_max = 0;    // dynamic cuboids per round
     unsigned long long split_fanout = 1;
     std::vector<std::uint32_t> st_subs, dyn_subs, dyn_initial;
@@ -245,13 +249,14 @@ namespace {
 enum : int { KIND_HULL_FINALIZE = 100, KIND_FINAL_NOSTATIC, KIND_INIT_HULL, KIND_HULL_DECODE, KIND_INITIAL_BOX,
              KIND_SET_COUNT };
 
-sqd_kernel_stat make_stat(int kind) {
+sqd_kernel_stat make_stat(int kind, bool fp32) {
     sqd_kernel_stat s{};
     static const char* modes[] = {"NONFINAL", "FINAL", "SPLIT", "INIT"};
     if(kind < 64) {
         s.mode = kind / 16;
         s.level = kind % 16;
-        std::snprintf(s.name, sizeof s.name, "%s<%s,%d>", s.mode <= 1 ? "static_level" : "dynamic_level",
+        std::snprintf(s.name, sizeof s.name, "%s<%s,%d>",
+                      s.mode <= 1 ? (fp32 ? "static_level_fp32" : "static_level") : "dynamic_level",
                       modes[s.mode], s.level);
     } else {
         s.mode = -1;
@@ -297,7 +302,7 @@ void sync(sqd_core* c) {
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
         auto it = c->stats.find(p.kind);
-        if(it == c->stats.end()) it = c->stats.emplace(p.kind, make_stat(p.kind)).first;
+        if(it == c->stats.end()) it = c->stats.emplace(p.kind, make_stat(p.kind, c->fp32)).first;
         it->second.launches += 1;
         it->second.milliseconds += ms;
     }
@@ -355,6 +360,9 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.block_counts = c->d_slots;
     a.consts = c->d_consts;
     a.generations = c->settings.generation_count;
+    a.undec = c->d_undec;
+    a.undec_count = nullptr;
+    a.undec_cap = c->undec_cap;
     return a;
 }
 
@@ -375,9 +383,13 @@ void run_static(sqd_core* c, int mode, const dev::BoxList& in, const unsigned lo
         dev::BoxList src = prev;
         if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
         dev::LevelArgs a = level_args(c, src, prev_base, prev_count, out);
+        a.undec_count = &c->d_state->undec_count[mode * 16 + L];
         const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
         launch(c, mode * 16 + L, "static level launch",
-               [&] { return c->ks->static_level(mode, L, a, children, c->cus, c->grid_override, c->stream); });
+               [&] {
+                   const int m = mode | (c->fp32 ? dev::kModeF32 : 0);
+                   return c->ks->static_level(m, L, a, children, c->cus, c->grid_override, c->stream);
+               });
         prev = out;
         prev_base = nullptr;
         prev_count = out.count;
@@ -600,6 +612,7 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->cus = static_cast<unsigned>(prop.multiProcessorCount);
         c->grid_override = s.grid_blocks;
+        c->fp32 = s.fp32_prefilter != 0;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         const auto& prog = program->prog;
         for(const auto& sp : prog.splits) {
@@ -625,7 +638,7 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         // kernels need at most k_max / 256 + 1 blocks)
         c->n_slots = std::max<unsigned long long>(8ull * c->cus, kmax / 256 + 2);
         if(c->grid_override > c->n_slots) c->grid_override = static_cast<unsigned>(c->n_slots);
-        c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * 4 * sizeof(unsigned long long);
+        c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * dev::kSlotWords * sizeof(unsigned long long);
         c->d_state = static_cast<DeviceState*>(static_cast<void*>(dalloc<unsigned long long>(c->state_bytes / 8)));
         c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
         void* hs = nullptr;
@@ -640,6 +653,10 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->crit.create(c->N, s.critical_capacity, false, false, &st->crit_count);
         c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false, &st->scratch_count);
         c->d_hull = dalloc<unsigned long long>(static_cast<std::size_t>(kmax) * c->N * 4);
+        if(c->fp32) {   // as many undecided children as a level buffer holds boxes
+            c->undec_cap = lvl_cap;
+            c->d_undec = dalloc<unsigned long long>(static_cast<std::size_t>(lvl_cap));
+        }
         c->d_has = dalloc<unsigned char>(static_cast<std::size_t>(kmax));
         c->d_consts = dalloc<double>(2 * static_cast<std::size_t>(std::max(1, c->ks->NC)));
         hip_check(hipMemcpy(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
@@ -661,6 +678,7 @@ void sqd_core_destroy(sqd_core* c) {
     for(auto& l : c->lvl) l.destroy();
     c->fin.destroy(); c->spl.destroy(); c->crit.destroy(); c->scratch.destroy();
     for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_state),
+                   static_cast<void*>(c->d_undec),
                    static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
         if(p) (void)hipFree(p);
     if(c->h_state) (void)hipHostFree(c->h_state);
@@ -825,6 +843,56 @@ int sqd_core_acos_fallbacks(sqd_core* c, uint64_t* n) {
         unsigned long long v = 0;
         hip_check(c->ks->acos_hard(&v), "acos counter");
         *n = v;
+    });
+}
+
+int sqd_core_fp32_stats(sqd_core* c, uint64_t* checked, uint64_t* decided) {
+    return guarded([&] {
+        if(!c || !checked || !decided) throw std::invalid_argument("null argument");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        *checked = c->h_state->f32_checked;
+        *decided = c->h_state->f32_decided;
+    });
+}
+
+int sqd_width_stats(sqd_core* c, const double* lb, const double* ub, size_t n, sqd_width_stat* out,
+                    size_t capacity, size_t* n_constraints) {
+    return guarded([&] {
+        if(!c || !n_constraints || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
+        const int nc = c->ks->NCONS;
+        *n_constraints = static_cast<size_t>(nc);
+        if(!out || nc == 0) return;
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        const std::size_t box_bytes = sizeof(double) * static_cast<std::size_t>(c->N) * n;
+        double *d_lb = nullptr, *d_ub = nullptr, *d_out = nullptr;
+        std::vector<double> h(3 * static_cast<std::size_t>(nc), 0.0);
+        auto release = [&] { for(double* q : {d_lb, d_ub, d_out}) if(q) (void)hipFree(q); };
+        try {
+            d_lb = dalloc<double>(std::max<std::size_t>(1, static_cast<std::size_t>(c->N) * n));
+            d_ub = dalloc<double>(std::max<std::size_t>(1, static_cast<std::size_t>(c->N) * n));
+            d_out = dalloc<double>(3 * static_cast<std::size_t>(nc));
+            hip_check(hipMemcpy(d_lb, lb, box_bytes, hipMemcpyHostToDevice), "upload boxes");
+            hip_check(hipMemcpy(d_ub, ub, box_bytes, hipMemcpyHostToDevice), "upload boxes");
+            hip_check(hipMemcpy(d_out, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "clear");
+            hip_check(c->ks->width_stats(d_lb, d_ub, n, c->d_consts, d_out, c->stream), "width statistics launch");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            hip_check(hipMemcpy(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost), "download");
+        } catch(...) { release(); throw; }
+        release();
+        for(int j = 0; j < nc && static_cast<size_t>(j) < capacity; ++j) {
+            sqd_width_stat& w = out[j];
+            w.constraint = c->ks->cons_id(j);
+            w.reserved = 0;
+            w.samples = static_cast<uint64_t>(h[3 * j + 1]);
+            w.mean_ratio = w.samples ? h[3 * j] / h[3 * j + 1] : 0.0;
+            std::uint64_t mb;
+            std::memcpy(&mb, &h[3 * j + 2], sizeof mb);
+            double mx;
+            std::memcpy(&mx, &mb, sizeof mx);
+            w.max_ratio = mx;
+        }
     });
 }
 

@@ -99,7 +99,7 @@ struct Emitter {
             auto sub = [&](int i) { Ctx c = scope; c.push_back({id, i}); return c; };
             if(n.op == FOp::Ite) {
                 const std::string c = b(n.kids[0]), t = "t" + std::to_string(id), e = "e" + std::to_string(id);
-                o << indent << "IV " << t << "{0.0, 0.0, false}, " << e << "{0.0, 0.0, false};\n";
+                o << indent << "IV " << t << "{}, " << e << "{};\n";
                 o << indent << "if(" << c << ".poss) {\n";
                 emit_scope(order, ctx, sub(1), indent + "    ", o);
                 o << indent << "    " << t << " = " << v(n.kids[1]) << ";\n" << indent << "}\n";
@@ -193,14 +193,41 @@ std::string generate(const flat::Program& p, const std::string& key) {
     std::map<int, bool> roots;
     for(const auto& r : p.rows) for(int c : r) roots[c] = true;
     for(const auto& kv : roots) {
-        body << "    __device__ static IB c" << kv.first << "(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
-        body << "        (void)lb; (void)ub; (void)C;\n";
+        body << "    template<class Num> __device__ static IB c" << kv.first
+             << "(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+        body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n";
         body << e.eval(kv.first, "        ");
         body << "        return " << e.b(kv.first) << ";\n    }\n";
     }
+    // per constraint: the interval of (left - right) of a top-level comparison,
+    // for the FP32-versus-FP64 width statistics (other roots: an empty interval)
+    std::vector<int> cons;
+    for(const auto& kv : roots) cons.push_back(kv.first);
+    for(int c : cons) {
+        const flat::FNode& n = p.nodes[static_cast<std::size_t>(c)];
+        const bool cmp = n.op == FOp::Lt || n.op == FOp::Le || n.op == FOp::Gt || n.op == FOp::Ge;
+        body << "    template<class Num> __device__ static IVT<Num> d" << c
+             << "(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+        body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n";
+        if(cmp) {
+            body << e.eval(c, "        ");
+            body << "        (void)" << e.b(c) << ";\n";
+            body << "        return iv_sub(" << e.v(n.kids[0]) << ", " << e.v(n.kids[1]) << ");\n    }\n";
+        } else {
+            body << "        return IV{Num(1), Num(0), true};\n    }\n";
+        }
+    }
+    body << "    static constexpr int NCONS = " << cons.size() << ";\n";
+    body << "    static constexpr int cons_id(int j) { switch(j) {";
+    for(std::size_t j = 0; j < cons.size(); ++j) body << " case " << j << ": return " << cons[j] << ";";
+    body << " default: return -1; } }\n";
+    body << "    template<class Num> __device__ static IVT<Num> cons_diff(int j, const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+    body << "        switch(j) {\n";
+    for(std::size_t j = 0; j < cons.size(); ++j) body << "        case " << j << ": return d" << cons[j] << "(lb, ub, C);\n";
+    body << "        default: return IVT<Num>{Num(1), Num(0), true};\n        }\n    }\n";
     // bound application
-    body << "    __device__ static int apply(int k, double (&lb)[N], double (&ub)[N], const double* C) {\n";
-    body << "        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";
+    body << "    template<class Num> __device__ static int apply(int k, Num (&lb)[N], Num (&ub)[N], const Num* C) {\n";
+    body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";
     for(std::size_t k = 0; k < p.bounds.size(); ++k) {
         const flat::FBound& bd = p.bounds[k];
         const int t = bd.target;
@@ -236,7 +263,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
     }
     body << "        default: break;\n        }\n    }\n";
     // constraint rows per static level
-    body << "    template<int L> __device__ static bool rows_violated(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
+    body << "    template<int L, class Num> __device__ static bool rows_violated(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
     body << "        (void)lb; (void)ub; (void)C;\n";
     for(std::size_t L = 0; L < sta.size(); ++L) {
         int from = p.splits[static_cast<std::size_t>(sta[L])].arg;
@@ -248,7 +275,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
         body << "        }\n";
     }
     body << "        return false;\n    }\n";
-    body << "    __device__ static bool all_satisfied(const double (&lb)[N], const double (&ub)[N], const double* C) {\n";
+    body << "    template<class Num> __device__ static bool all_satisfied(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
     for(int c : p.rows[static_cast<std::size_t>(p.nargs - 1)]) body << "        if(!c" << c << "(lb, ub, C).poss) return false;\n";
     body << "        (void)lb; (void)ub; (void)C;\n        return true;\n    }\n";
 

@@ -37,7 +37,12 @@ class Settings(ctypes.Structure):
     _fields_ = [("generation_count", ctypes.c_uint32), ("dequeue_buffer_size", ctypes.c_uint32),
                 ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
                 ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("fp32_prefilter", ctypes.c_uint32)]
+
+
+class WidthStat(ctypes.Structure):
+    _fields_ = [("constraint", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("samples", ctypes.c_uint64),
+                ("mean_ratio", ctypes.c_double), ("max_ratio", ctypes.c_double)]
 
 
 class KernelStat(ctypes.Structure):
@@ -88,6 +93,8 @@ def lib():
             "sqd_interval_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
             "sqd_test_acos_fallbacks": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_core_acos_fallbacks": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_core_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_width_stats": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(WidthStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_last_error": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -105,7 +112,8 @@ def exported_symbols():
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_criticals",
             "sqd_core_set_profiling",
-            "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_rounded_op", "sqd_interval_op",
+            "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
+            "sqd_rounded_op", "sqd_interval_op",
             "sqd_test_acos_fallbacks", "sqd_last_error"]
 
 
@@ -271,6 +279,27 @@ class Core:
         _check(lib().sqd_core_acos_fallbacks(self._h, ctypes.byref(n)))
         return int(n.value)
 
+    def fp32_stats(self):
+        """(checked, decided): static-level children of the last call that the FP32
+        prefilter evaluated and pruned without binary64."""
+        a = ctypes.c_uint64()
+        b = ctypes.c_uint64()
+        _check(lib().sqd_core_fp32_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
+
+    def width_stats(self, lb, ub):
+        """Per constraint: FP32-versus-FP64 width ratio of its compared difference
+        over the boxes (SoA arrays of shape (N, n))."""
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        k = _SZ()
+        _check(lib().sqd_width_stats(self._h, _dptr(lb), _dptr(ub), n, None, 0, ctypes.byref(k)))
+        arr = (WidthStat * max(1, k.value))()
+        _check(lib().sqd_width_stats(self._h, _dptr(lb), _dptr(ub), n, arr, k.value, ctypes.byref(k)))
+        return [{"constraint": arr[i].constraint, "samples": int(arr[i].samples), "mean_ratio": arr[i].mean_ratio,
+                 "max_ratio": arr[i].max_ratio} for i in range(k.value)]
+
     def set_profiling(self, enable=True, only=None):
         """Time launches with HIP events on the core's stream (clears the statistics).
 
@@ -289,7 +318,8 @@ class Core:
 
 def rounded_op(op, a, b, device=0):
     """Device RD/RU of a op b (op: 'add', 'sub', 'mul', 'div', 'sqrt')."""
-    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5}[op]
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5, "add32": 8, "sub32": 9, "mul32": 10,
+            "div32": 11, "sqrt32": 12, "acos32": 13, "to32": 14}[op]
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     lo = np.zeros_like(a)

@@ -8,6 +8,7 @@ Steps:
   3. link libsqd.so next to this file;
   4. build the CPU oracle (oracle/Makefile) for the tests and the CPU baseline.
 """
+import hashlib
 import os
 import shutil
 import subprocess
@@ -49,7 +50,20 @@ def _inc():
     return ["-I" + str(p) for p in INCLUDES]
 
 
-def build(verbose=False, jobs=None):
+def _stamp(cmd, src):
+    """Hash of a compile: the command, the source and every header it may include."""
+    h = hashlib.sha256(" ".join(str(c) for c in cmd).encode())
+    h.update(Path(src).read_bytes())
+    for d in (PKG / "csrc", ROOT / "include", PKG / "host" / "ivarp", PKG / "lemmas"):
+        for f in sorted(d.glob("*.h*")):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()
+
+
+def build(verbose=False, jobs=None, incremental=False):
+    """Compile everything; with incremental=True an object whose command, source
+    and headers are unchanged since its last compile is reused."""
     jobs = jobs or max(1, min(16, os.cpu_count() or 1))
     BUILD.mkdir(exist_ok=True)
     GEN.mkdir(exist_ok=True)
@@ -65,16 +79,28 @@ def build(verbose=False, jobs=None):
     # 2. objects
     objs = []
     tasks = []
+    def add(cmd, src, o):
+        objs.append(o)
+        st = o.with_suffix(".stamp")
+        key = _stamp(cmd, src)
+        if incremental and o.exists() and st.exists() and st.read_text() == key:
+            return
+        if st.exists():
+            st.unlink()
+        tasks.append((cmd, st, key))
     for s in HOST_SOURCES:
         o = BUILD / (s.stem + ".host.o")
-        objs.append(o)
-        tasks.append([CXX] + CXXFLAGS + _inc() + ["-c", str(s), "-o", str(o)])
+        add([CXX] + CXXFLAGS + _inc() + ["-c", str(s), "-o", str(o)], s, o)
     for s in HIP_SOURCES + gen_hip:
         o = BUILD / (s.stem + ".hip.o")
-        objs.append(o)
-        tasks.append([HIPCC] + HIPFLAGS + _inc() + ["-c", str(s), "-o", str(o)])
+        add([HIPCC] + HIPFLAGS + _inc() + ["-c", str(s), "-o", str(o)], s, o)
+
+    def compile_one(task):
+        cmd, st, key = task
+        _run(cmd, verbose)
+        st.write_text(key)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        for f in [ex.submit(_run, t, verbose) for t in tasks]:
+        for f in [ex.submit(compile_one, t) for t in tasks]:
             f.result()
     # 3. link
     tmp = LIB.with_suffix(".so.tmp")
@@ -86,5 +112,5 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv)
     print("built", LIB)

@@ -219,3 +219,69 @@ def test_oracle_acos_interval_domain():
     _, hipi = pyoracle.rounded_op("acos", [-1.0], [0.0])
     assert (r[0][0], r[0][1], r[0][2]) == (lo05[0], hipi[0], 1.0)
     assert r[1][2] == 0.0 and r[1][0] < r[1][1]
+
+
+# ---------------------------------------------------------------- binary32 (FP32 prefilter)
+def _rd32(fr):
+    """Largest float32 <= the rational fr (finite range)."""
+    c = np.float32(float(fr))
+    while Fraction(float(c)) > fr:
+        c = np.nextafter(c, np.float32(-np.inf))
+    while Fraction(float(np.nextafter(c, np.float32(np.inf)))) <= fr:
+        c = np.nextafter(c, np.float32(np.inf))
+    return float(c)
+
+
+def _ru32(fr):
+    c = np.float32(float(fr))
+    while Fraction(float(c)) < fr:
+        c = np.nextafter(c, np.float32(np.inf))
+    while Fraction(float(np.nextafter(c, np.float32(-np.inf)))) >= fr:
+        c = np.nextafter(c, np.float32(-np.inf))
+    return float(c)
+
+
+def _random_floats(rng, n):
+    m = rng.uniform(0.5, 1.0, n) * np.exp2(rng.integers(-40, 40, n).astype(float))
+    return (m * rng.choice([-1.0, 1.0], n)).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div", "sqrt"])
+def test_oracle_binary32_directed_rounding_exact(op):
+    rng = np.random.default_rng(31 + len(op))
+    a = _random_floats(rng, 2000)
+    b = _random_floats(rng, 2000)
+    if op == "sqrt":
+        a = np.abs(a)
+    lo, hi = pyoracle.rounded_op(op + "32", a, b)
+    for x, y, l, h in zip(a, b, lo, hi):
+        fx, fy = Fraction(float(x)), Fraction(float(y))
+        if op == "sqrt":
+            s = _rd32(Fraction(math.sqrt(float(x))))          # candidate, then exact check
+            while Fraction(s) ** 2 > fx:
+                s = float(np.nextafter(np.float32(s), np.float32(-np.inf)))
+            while Fraction(float(np.nextafter(np.float32(s), np.float32(np.inf)))) ** 2 <= fx:
+                s = float(np.nextafter(np.float32(s), np.float32(np.inf)))
+            exp_l = s
+            exp_h = s if Fraction(s) ** 2 == fx else float(np.nextafter(np.float32(s), np.float32(np.inf)))
+        else:
+            e = {"add": fx + fy, "sub": fx - fy, "mul": fx * fy, "div": fx / fy}[op]
+            exp_l, exp_h = _rd32(e), _ru32(e)
+        assert (l, h) == (exp_l, exp_h), (op, x, y)
+
+
+def test_oracle_binary32_outward_conversion_and_acos():
+    rng = np.random.default_rng(77)
+    x = np.concatenate([rng.uniform(-2, 2, 3000), [0.1, -0.1, 1.0 / 3, 1e-40, -1e-40, 3.5e38, -3.5e38]])
+    lo, hi = pyoracle.rounded_op("to32", x, x)
+    fmax = float(np.finfo(np.float32).max)
+    for v, l, h in zip(x[:-2], lo[:-2], hi[:-2]):
+        fr = Fraction(float(v))
+        assert (l, h) == (_rd32(fr), _ru32(fr)), v
+    assert (lo[-2], hi[-2]) == (fmax, INF) and (lo[-1], hi[-1]) == (-INF, -fmax)   # beyond the float range
+    # binary32 acos = binary64 directed results converted outward: adjacent floats around acos(x)
+    xs = rng.uniform(-1, 1, 2000).astype(np.float32).astype(np.float64)
+    lo32, hi32 = pyoracle.rounded_op("acos32", xs, xs)
+    lo64, hi64 = pyoracle.rounded_op("acos", xs, xs)
+    assert np.all(lo32 <= lo64) and np.all(hi32 >= hi64)
+    assert np.array_equal(np.nextafter(lo32.astype(np.float32), np.float32(np.inf)).astype(np.float64), hi32)

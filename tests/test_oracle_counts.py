@@ -90,3 +90,16 @@ def test_oracle_core_counts_direct(program_text):
         cl, cu, cf = o.final(lb, ub, cap=n * 64)
         assert cf["num_critical_cuboids"] == n * 64 and cf["num_leaf_cuboids"] == n * 64
         assert cf["num_cuboids"] == n * 8 + n * 8 * 8
+
+
+@pytest.mark.parametrize("key", ["one", "three"])
+def test_oracle_fp32_prefilter_keeps_published_counts(program_text, key):
+    # With the FP32 prefilter the oracle's counts stay the published ones (every
+    # child binary32 prunes is one binary64 prunes too, for these statements) and
+    # binary32 decides the large majority of static-level children.
+    o = pyoracle.Oracle(program_text(key), fp32=True)
+    counts, _ = o.run(threads=8)
+    assert (counts["num_cuboids"], counts["num_leaf_cuboids"]) == PUBLISHED[key]
+    assert counts["num_critical_cuboids"] == 0
+    checked, decided = o.fp32_stats()
+    assert checked > 0 and 0.95 < decided / checked <= 1.0
