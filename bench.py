@@ -20,11 +20,25 @@ checked against the published ones.  The line printed carries
               the MI355X FP64 vector peak
   cpu_baseline the CPU restatement (oracle) timed on this host's cores on the
               same proof
+  fp32_first_batched
+              the same proofs through the proof service: the program's settings
+              autotuned (FP32 prefilter levels, cuboids per round, blocks per
+              launch), `--workers` proofs in flight on their own streams; its
+              cuboids/s, p50/p99 job latency and ratio to `value`, the fraction
+              of static-level children the FP32 pass decided, and the
+              FP32-versus-FP64 width ratio of each constraint on sampled boxes
+  job_stream  proof jobs over all 11 statements (statement uniform, a
+              contiguous 1/2^k slice of the first variable's initial
+              subdivisions, k uniform in 0..6, seeded), first run as one batch to
+              measure capacity, then submitted with Poisson arrivals at 70 % of
+              it: p50/p99 latency and cuboids/s; every job's counts are checked
+              against the sum of its subtrees' (a sequential binary64 run)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one] [--workers 8]
 """
 import argparse
 import json
+import random
 import os
 import statistics
 import sys
@@ -61,6 +75,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=None,
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
+    ap.add_argument("--workers", type=int, default=8, help="proofs in flight in the batched legs")
+    ap.add_argument("--batched-proofs", type=int, default=256)
+    ap.add_argument("--stream-jobs", type=int, default=48)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB: a kernel of known traffic that "
                          "scales the profiler's FETCH_SIZE/WRITE_SIZE readings")
@@ -120,6 +140,20 @@ def cpu_baseline(program_text, runs):
     }
 
 
+def fp32_op_counts(program_text):
+    """Binary64 and binary32 operation counts per phase of the FP32-prefiltered proof (oracle)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.NamedTemporaryFile("w", suffix=".prog", delete=False) as f:
+        f.write(program_text)
+        path = f.name
+    try:
+        return pyoracle.run_cli(path, threads=threads, flops=True, fp32=True, timeout=600)
+    finally:
+        os.unlink(path)
+
+
 def read_traffic(path, kernel_name):
     """Bytes per launch of `kernel_name` from a traffic summary (key = value lines)."""
     if not path or not Path(path).exists():
@@ -132,6 +166,133 @@ def read_traffic(path, kernel_name):
     if vals.get("kernel") != kernel_name or "bytes_per_launch" not in vals:
         return None
     return float(vals["bytes_per_launch"])
+
+
+LEMMAS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven", "more1", "more2"]
+
+
+def initial_subdivisions(program):
+    """The first dynamic variable's initial subdivisions (from the program text)."""
+    for line in program.text().splitlines():
+        f = line.split()
+        if f and f[0] == "split" and f[4] == "1":
+            return int(f[2])
+    return 1
+
+
+def fp32_first_batched(sqd, prog, expected, workers, proofs, plain_value, counted32):
+    """The batched FP32-first leg on `prog` (full proofs through the service)."""
+    best, tune = sqd.autotune(prog)
+    tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "dequeue_buffer_size", "grid_blocks")}
+    n = initial_subdivisions(prog)
+    # decided fraction and the dominant prefiltered kernel's time on one core
+    with sqd.Core(prog, settings=best) as core:
+        core.prove()
+        core.set_profiling(True)
+        for _ in range(20):
+            core.prove()
+        ks = core.kernel_stats()
+        checked, decided = core.fp32_stats()
+        core.set_profiling(False)
+    with sqd.Service(workers=workers, autotune=1) as svc:
+        for _ in range(2 * workers):                          # tunes once, then warms every worker's core
+            svc.wait(svc.submit(prog, 0, n))
+        t0 = time.perf_counter()
+        ids = [svc.submit(prog, 0, n) for _ in range(proofs)]
+        res = [svc.wait(i) for i in ids]
+        elapsed = time.perf_counter() - t0
+    for r in res:
+        c = r["counts"]
+        if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+            raise SystemExit("count mismatch in the batched leg: %s" % c)
+    lat = [r["latency_seconds"] for r in res]
+    value = sum(r["counts"]["num_cuboids"] for r in res) / elapsed
+    out = {
+        "value": value, "unit": "cuboids/s", "ratio_vs_plain_fp64": value / plain_value,
+        "workers": workers, "proofs": proofs, "seconds": elapsed,
+        "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
+        "run_ms": {"p50": percentile([r["run_seconds"] for r in res], 0.5) * 1e3},
+        "autotuned": tuned, "autotune_slice": [tune["first"], tune["last"]],
+        "fp32_checked": checked, "fp32_decided": decided,
+        "fp32_decided_fraction": decided / checked if checked else None,
+        "single_core_kernel_ms_per_proof": {s["name"]: s["milliseconds"] / 20 for s in
+                                            sorted(ks, key=lambda s: -s["milliseconds"])[:4]},
+    }
+    if counted32 is not None:
+        out["fp32_ops_per_proof"] = {k: v for k, v in counted32.get("phases", {}).items()}
+    return out
+
+
+def width_sample(sqd, prog, n_boxes=8192, seed=3):
+    """FP32-versus-FP64 width ratio per constraint on boxes of the static grid."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    with sqd.Core(prog) as core:
+        qlb, qub, _ = core.initial_queue()
+        pick = rng.integers(0, qlb.shape[1], n_boxes)
+        lb, ub = qlb[:, pick].copy(), qub[:, pick].copy()
+        for line in prog.text().splitlines():                 # one random cell of every static variable
+            f = line.split()
+            if f and f[0] == "split" and f[4] == "0":
+                a, subs = int(f[1]), int(f[3])
+                cell = rng.integers(0, subs, n_boxes)
+                w = (ub[a] - lb[a]) / subs
+                lb[a], ub[a] = lb[a] + cell * w, lb[a] + (cell + 1) * w
+        st = core.width_stats(lb, ub)
+    return {"c%d" % s["constraint"]: {"mean_ratio": round(s["mean_ratio"], 4), "max_ratio": round(s["max_ratio"], 4),
+                                       "samples": s["samples"]} for s in st if s["samples"]}
+
+
+def job_stream(sqd, jobs_n, workers, seed):
+    """Poisson stream of proof jobs over all statements (see the module docstring)."""
+    rng = random.Random(seed)
+    progs = {k: sqd.Program.from_lemma(k) for k in LEMMAS}
+    init = {k: initial_subdivisions(p) for k, p in progs.items()}
+    jobs = []
+    for _ in range(jobs_n):
+        k = rng.choice(LEMMAS)
+        e = rng.randint(0, 6)
+        size = init[k] >> e
+        first = rng.randrange(0, init[k] // size) * size
+        jobs.append((k, first, first + size))
+    # reference counts: each job alone, one at a time, binary64
+    alone = {}
+    with sqd.Service(workers=1, fp32_prefilter=0) as svc:
+        for j in set(jobs):
+            alone[j] = svc.wait(svc.submit(progs[j[0]], j[1], j[2]))["counts"]
+    with sqd.Service(workers=workers, autotune=1) as svc:
+        for k in LEMMAS:                                      # tune and create a core per statement
+            svc.wait(svc.submit(progs[k], 0, 1))
+        t0 = time.perf_counter()
+        ids = [svc.submit(progs[k], a, b) for k, a, b in jobs]
+        batch = [svc.wait(i) for i in ids]
+        makespan = time.perf_counter() - t0
+        rate = 0.7 * len(jobs) / makespan                     # 70 % of the measured capacity
+        arrivals, t = [], 0.0
+        for _ in jobs:
+            t += rng.expovariate(rate)
+            arrivals.append(t)
+        t0 = time.perf_counter()
+        ids = []
+        for (k, a, b), at in zip(jobs, arrivals):
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids.append(svc.submit(progs[k], a, b))
+        res = [svc.wait(i) for i in ids]
+        elapsed = time.perf_counter() - t0
+    for j, r, rb in zip(jobs, res, batch):
+        if r["counts"] != alone[j] or rb["counts"] != alone[j] or r["n_critical"]:
+            raise SystemExit("job stream: %s differs from its sequential run" % (j,))
+    lat = [r["latency_seconds"] for r in res]
+    cub = sum(r["counts"]["num_cuboids"] for r in res)
+    return {
+        "jobs": len(jobs), "seed": seed, "workers": workers, "arrival_rate_per_s": rate,
+        "capacity_jobs_per_s": len(jobs) / makespan,
+        "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
+        "cuboids_per_s": cub / elapsed, "cuboids": cub, "seconds": elapsed,
+        "same_as_alone": True,
+    }
 
 
 def main():
@@ -175,16 +336,23 @@ def main():
     finally:
         core.close()
 
-    counted, cpu = (None, None)
+    counted, cpu, counted32 = (None, None, None)
     if not a.no_cpu_baseline:
         counted, cpu = cpu_baseline(prog.text(), a.cpu_runs)
+        counted32 = fp32_op_counts(prog.text())
+    value = total_cuboids / elapsed
+    batched = None if a.no_batched else fp32_first_batched(sqd, prog, expected, a.workers, a.batched_proofs, value,
+                                                           counted32)
+    if batched is not None:
+        batched["width_ratio_per_constraint"] = width_sample(sqd, prog)
+    stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.seed)
 
     dom = next(s for s in stats if s["mode"] == dom["mode"] and s["level"] == dom["level"])
     mean_ms = dom["milliseconds"] / dom["launches"]
     roofline = None
     if counted is not None:
-        ph = counted["phases"].get("%d.%d" % (dom["mode"], dom["level"]), [0, 0])
-        flops_per_proof, cuboids_per_proof = ph
+        ph = counted["phases"].get("%d.%d" % (dom["mode"], dom["level"]), [0, 0, 0])
+        flops_per_proof, cuboids_per_proof = ph[0], ph[1]
         launches_per_proof = dom["launches"] / a.steps
         flops_per_launch = flops_per_proof / launches_per_proof
         achieved = flops_per_launch / (mean_ms * 1e-3) / 1e12
@@ -199,7 +367,7 @@ def main():
         }
     line = {
         "metric": METRIC,
-        "value": total_cuboids / elapsed,
+        "value": value,
         "unit": "cuboids/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -218,6 +386,8 @@ def main():
                                       sorted(warm, key=lambda s: -s["milliseconds"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32_first_batched": batched,
+        "job_stream": stream,
     }
     print(json.dumps(line), flush=True)
     if a.copy_calibration:

@@ -60,13 +60,21 @@ typedef struct sqd_settings {
     uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
     uint32_t grid_blocks;          /* blocks per level launch; default: the kernel's resident
                                       capacity (occupancy x CUs), capped by the work */
-    uint32_t fp32_prefilter;       /* nonzero: static levels first decide each child in
-                                      outward-rounded binary32 and re-evaluate only the
-                                      undecided ones in binary64 (no counterpart in the
-                                      reference, which evaluates everything in binary64) */
+    uint32_t fp32_prefilter;       /* bit mask of the static levels (bit L: level L) that first
+                                      decide each child in outward-rounded binary32 and
+                                      re-evaluate only the undecided ones in binary64;
+                                      0 = off, SQD_FP32_ALL_LEVELS = every level (no
+                                      counterpart in the reference, which evaluates
+                                      everything in binary64) */
+    uint32_t graphs;               /* nonzero: sqd_prove replays its rounds as captured HIP
+                                      graphs from the second proof on a core (default 1) */
+    uint32_t reserved;
 } sqd_settings;
 
-/* Defaults: generation_count = 8, every sizing field 0 (chosen by the core). */
+#define SQD_FP32_ALL_LEVELS 0xffffffffu
+
+/* Defaults: generation_count = 8, graphs = 1, every other field 0 (sizes chosen
+ * by the core, no FP32 prefilter). */
 void sqd_settings_default(sqd_settings* settings);
 
 /* ---------------------------------------------------------------- programs */
@@ -95,6 +103,9 @@ const char* sqd_lemma_title(int i);
  * gpu_prover_core.cuh:241-263,427-464): allocate device state on `device`. */
 int sqd_core_create(int device, const sqd_program* program, const sqd_settings* settings, sqd_core** out);
 void sqd_core_destroy(sqd_core* core);
+/* Device memory the core allocated (frontier, level buffers, lists, state).
+ * No counterpart in the reference. */
+int sqd_core_memory(const sqd_core* core, uint64_t* device_bytes);
 
 /* Initial runtime box (refactor_constraints_impl.hpp:61-101): N lb then N ub. */
 int sqd_initial_box(sqd_core* core, double* lb, double* ub);
@@ -124,6 +135,14 @@ int sqd_split(sqd_core* core, const double* lb, const double* ub, size_t n,
  * spent on the device loop. */
 int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_ub,
               size_t capacity, size_t* n_critical, double* seconds);
+
+/* The same proof restricted to the initial subdivisions [first, last) of the
+ * first dynamic variable (proof_driver.hpp:320-391 splits it into `initial`
+ * parts first): the exact union of those subtrees of the full proof, so the
+ * counts of a partition of [0, initial) add up to sqd_prove's.  The unit of work
+ * of the proof service.  No counterpart in the reference. */
+int sqd_prove_range(sqd_core* core, uint32_t first, uint32_t last, sqd_counts* counts, double* crit_lb,
+                    double* crit_ub, size_t capacity, size_t* n_critical, double* seconds);
 
 /* The critical cuboids found by the last sqd_prove on this core (they stay in
  * device memory until the next call): up to `capacity` boxes (SoA, stride
@@ -162,6 +181,76 @@ typedef struct sqd_width_stat {
 } sqd_width_stat;
 int sqd_width_stats(sqd_core* core, const double* lb, const double* ub, size_t n, sqd_width_stat* out,
                     size_t capacity, size_t* n_constraints);
+
+/* The number of initial subdivisions of the first dynamic variable (the range
+ * sqd_prove_range selects from) and the number of static levels (the bits of
+ * sqd_settings.fp32_prefilter that apply).  No counterpart in the reference. */
+int sqd_core_shape(const sqd_core* core, uint32_t* initial_subdivisions, uint32_t* static_levels);
+
+/* ---------------------------------------------------------------- autotuner */
+
+/* Chooses, by coordinate descent over timed sqd_prove_range runs on a slice of
+ * the program's first variable, the FP32-prefilter level mask, the dynamic
+ * cuboids per round (dequeue_buffer_size) and the blocks per level launch
+ * (grid_blocks), starting from `base` (NULL: the defaults).  Candidates whose
+ * counts differ from the plain binary64 run are rejected.  No counterpart in the
+ * reference. */
+typedef struct sqd_autotune_options {
+    double target_seconds;    /* slice of the first variable to time: about this long (default 0.02) */
+    int32_t repetitions;      /* timed proofs per candidate, median taken (default 3) */
+    uint32_t reserved;
+} sqd_autotune_options;
+typedef struct sqd_autotune_report {
+    uint32_t first, last;     /* the slice [first, last) that was timed */
+    uint32_t trials;          /* candidates timed */
+    uint32_t reserved;
+    double plain_seconds;     /* the base settings without the prefilter */
+    double best_seconds;      /* the chosen settings */
+    sqd_counts counts;        /* counts of the slice (equal for every accepted candidate) */
+} sqd_autotune_report;
+void sqd_autotune_options_default(sqd_autotune_options* options);
+int sqd_autotune(int device, const sqd_program* program, const sqd_settings* base, const sqd_autotune_options* options,
+                 sqd_settings* best, sqd_autotune_report* report);
+
+/* ---------------------------------------------------------------- proof service */
+
+/* A queue of proof jobs (a program and a range [first, last) of its first
+ * variable's initial subdivisions, as in sqd_prove_range) run by worker threads,
+ * each on a pooled core of the job's program with its own stream, so that the
+ * kernels of up to `workers` jobs overlap on the device.  Cores are created on
+ * demand within `memory_budget` bytes of device memory; idle cores of other
+ * programs are destroyed to make room.  Jobs may be submitted from any thread
+ * while others run.  Programs must outlive the service.  No counterpart in the
+ * reference (ProofDriver runs one proof at a time, prover/proof_driver.hpp:393-438). */
+typedef struct sqd_service sqd_service;
+typedef struct sqd_service_options {
+    int32_t device;
+    uint32_t workers;             /* jobs in flight (default 8) */
+    uint64_t memory_budget;       /* device bytes for the pooled cores (default 64 GiB) */
+    uint32_t fp32_prefilter;      /* prefilter level mask of every core (default SQD_FP32_ALL_LEVELS) */
+    uint32_t autotune;            /* nonzero: sqd_autotune each program on its first job */
+    uint64_t level_bytes;         /* per core (default 256 MiB) */
+    uint64_t frontier_capacity;   /* per core (default 1 << 22 cuboids) */
+    uint64_t core_bytes_estimate; /* size assumed for a program's first core (default 2 GiB) */
+} sqd_service_options;
+typedef struct sqd_job_result {
+    sqd_counts counts;
+    uint64_t n_critical;
+    double queued_seconds;        /* submission to start */
+    double run_seconds;           /* start to completion */
+    double latency_seconds;       /* submission to completion */
+    int32_t status;               /* SQD_OK or the job's error */
+    uint32_t reserved;
+} sqd_job_result;
+void sqd_service_options_default(sqd_service_options* options);
+int sqd_service_create(const sqd_service_options* options, sqd_service** out);
+int sqd_service_submit(sqd_service* service, const sqd_program* program, uint32_t first, uint32_t last, uint64_t* job);
+/* *done = 1 once the job has finished (its result is then copied to *out). */
+int sqd_service_poll(sqd_service* service, uint64_t job, int* done, sqd_job_result* out);
+/* Blocks until the job finishes, copies its result and forgets the job; returns its status. */
+int sqd_service_wait(sqd_service* service, uint64_t job, sqd_job_result* out);
+/* Stops the workers (running jobs finish, queued ones are dropped) and frees the cores. */
+void sqd_service_destroy(sqd_service* service);
 
 /* ---------------------------------------------------------------- timing */
 

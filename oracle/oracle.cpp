@@ -708,6 +708,8 @@ struct Core {
         enumerate(ev, e, 0, 1, cnt, [&](const Cuboid& leaf) { cnt.criticals++; crit(leaf); }, ev32);
     }
     // Dynamic split (pd_impl_split_into.hpp:29-75) or initial split (proof_driver.hpp:320-391).
+    // initial subdivisions of the first dynamic variable the initial split enumerates
+    std::uint64_t init_lo = 0, init_hi = ~0ull;
     template<typename Out>
     void split(Evaluator& ev, const Cuboid& e, std::size_t level, bool initial, Counts& cnt, Out&& out) const {
         const std::size_t itlim = 2 * static_cast<std::size_t>(p.N);
@@ -721,7 +723,10 @@ struct Core {
         const int mode = initial ? 3 : 2;
         enter_phase(mode, level, static_cast<std::uint64_t>(n));
         Splitter sp(e.lb[static_cast<std::size_t>(s.arg)], e.ub[static_cast<std::size_t>(s.arg)], n);
-        for(int i = 0; i < n; ++i) {
+        const bool ranged = initial && level == 0;
+        const int i0 = ranged ? static_cast<int>(std::min<std::uint64_t>(init_lo, static_cast<std::uint64_t>(n))) : 0;
+        const int i1 = ranged ? static_cast<int>(std::min<std::uint64_t>(init_hi, static_cast<std::uint64_t>(n))) : n;
+        for(int i = i0; i < i1; ++i) {
             enter_phase(mode, level, 0);
             cnt.cuboids++;
             Cuboid q = e;
@@ -745,8 +750,10 @@ Cuboid initial_box(const Program& p) {
     return c;
 }
 
-std::vector<Cuboid> initial_queue(const Program& p, Counts& cnt) {
+std::vector<Cuboid> initial_queue(const Program& p, Counts& cnt, std::uint64_t lo = 0, std::uint64_t hi = ~0ull) {
     Core core(p);
+    core.init_lo = lo;
+    core.init_hi = hi;
     Evaluator ev(p);
     std::vector<Cuboid> q;
     Cuboid root = initial_box(p);
@@ -828,6 +835,7 @@ thread_local std::string t_error;
 
 struct Handle {
     Program prog;
+    std::uint64_t init_lo = 0, init_hi = ~0ull;   // initial range of the first dynamic variable
     bool fp32 = false;       // FP32 prefilter in the static levels
     Counts last;             // counts of the last computing call
 };
@@ -898,7 +906,8 @@ int oracle_run(void* handle, int generations, int threads, OracleCounts* counts,
         RoundDownGuard g;
         const Program& p = static_cast<Handle*>(handle)->prog;
         Counts init;
-        std::vector<Cuboid> q = initial_queue(p, init);
+        Handle* hr = static_cast<Handle*>(handle);
+        std::vector<Cuboid> q = initial_queue(p, init, hr->init_lo, hr->init_hi);
         // the reference pops the last element first; reverse-free order does not change counts
         RunResult r;
         Handle* h = static_cast<Handle*>(handle);
@@ -1104,6 +1113,13 @@ int oracle_phase_counts32(int mode, int level, std::uint64_t* flops) {
     if(mode < 0 || mode > 4 || level < 0 || level >= kPhaseLevels || !flops) return -1;
     const int ph = mode == 4 ? kPhaseOther : mode * kPhaseLevels + level;
     *flops = g_phase_flops32[ph].load();
+    return 0;
+}
+
+int oracle_set_init_range(void* handle, std::uint64_t first, std::uint64_t last) {
+    if(!handle || first >= last) return -1;
+    static_cast<Handle*>(handle)->init_lo = first;
+    static_cast<Handle*>(handle)->init_hi = last;
     return 0;
 }
 

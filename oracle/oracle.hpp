@@ -99,6 +99,9 @@ int oracle_phase_counts32(int mode, int level, std::uint64_t* flops);
 // decided in binary64 as usual.  Applies to oracle_run, oracle_run_slice,
 // oracle_nonfinal and oracle_final.
 int oracle_set_fp32_prefilter(void* handle, int on);
+// oracle_run from the initial subdivisions [first, last) of the first dynamic
+// variable only (the proof service's unit of work; no counterpart in the reference).
+int oracle_set_init_range(void* handle, std::uint64_t first, std::uint64_t last);
 // Static-level children of the last computing call that binary32 evaluated / pruned.
 int oracle_fp32_stats(void* handle, std::uint64_t* checked, std::uint64_t* decided);
 }

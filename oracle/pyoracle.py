@@ -65,6 +65,7 @@ def lib():
             "oracle_phase_counts": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
             "oracle_phase_counts32": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_U64)]),
             "oracle_set_fp32_prefilter": (ctypes.c_int, [_P, ctypes.c_int]),
+            "oracle_set_init_range": (ctypes.c_int, [_P, _U64, _U64]),
             "oracle_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
         }
         for name, (res, args) in sig.items():
@@ -91,6 +92,10 @@ class Oracle:
         self.N = lib().oracle_nargs(self._h)
         if fp32:
             _check(lib().oracle_set_fp32_prefilter(self._h, 1))
+
+    def set_init_range(self, first, last):
+        """Restrict run() to the initial subdivisions [first, last) of the first dynamic variable."""
+        _check(lib().oracle_set_init_range(self._h, first, last))
 
     def fp32_stats(self):
         """(checked, decided) static-level children of the last call under the FP32 prefilter."""

@@ -85,6 +85,9 @@ struct LevelArgs {
     unsigned long long* undec;
     unsigned long long* undec_count;
     unsigned long long undec_cap;
+    // initial queue: the first dynamic variable's initial subdivisions [range[0], range[1])
+    // (nullptr: all of them)
+    const unsigned long long* init_range;
 };
 
 // ---------------------------------------------------------------- hull keys
@@ -486,7 +489,11 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
-    const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
+    // the initial queue may start from a range of the first variable's subdivisions
+    constexpr bool RANGED = (MODE == INIT && J == 0);
+    const unsigned long long r_lo = RANGED && args.init_range ? args.init_range[0] : 0ull;
+    const unsigned long long r_n = RANGED && args.init_range ? args.init_range[1] - r_lo : static_cast<unsigned long long>(SUBS);
+    const unsigned long long total = n_items * r_n;
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const unsigned long long trips = (total + stride - 1) / stride;
@@ -497,8 +504,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
         double lb[N], ub[N];
         unsigned depth = 0;
         if(w < total) {
-            const unsigned long long item = w / SUBS;
-            const int i = static_cast<int>(w % SUBS);
+            const unsigned long long item = w / r_n;
+            const int i = static_cast<int>(r_lo + w % r_n);
             const unsigned long long src = in_base + item;
             load_box<P>(args.in, src, lb, ub);
             depth = args.in.depth ? args.in.depth[src] : 0u;

@@ -36,7 +36,7 @@ struct KernelSet {
     hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
                               dev::BoxList out, unsigned char* has, hipStream_t s);
     // Undecided acos roundings of this program so far (device-wide counter).
-    hipError_t (*acos_hard)(unsigned long long* out);
+    hipError_t (*acos_hard)(unsigned long long* out, hipStream_t s);
     // Constraints (roots of the constraint rows) and the FP32-versus-FP64 width
     // statistics of their compared differences over n boxes (k_width_stats).
     int NCONS;
@@ -168,8 +168,9 @@ hipError_t width_stats(const double* lb, const double* ub, unsigned long long n,
     return hipGetLastError();
 }
 
-inline hipError_t acos_hard(unsigned long long* out) {
-    return hipMemcpyFromSymbol(out, HIP_SYMBOL(dev::g_acos_hard), sizeof(unsigned long long), 0, hipMemcpyDeviceToHost);
+inline hipError_t acos_hard(unsigned long long* out, hipStream_t s) {
+    return hipMemcpyFromSymbolAsync(out, HIP_SYMBOL(dev::g_acos_hard), sizeof(unsigned long long), 0,
+                                    hipMemcpyDeviceToHost, s);
 }
 
 } // namespace detail

@@ -78,6 +78,9 @@ static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
+__global__ void k_set_range(unsigned long long* p, unsigned long long lo, unsigned long long hi) {
+    if(threadIdx.x == 0 && blockIdx.x == 0) { p[0] = lo; p[1] = hi; }
+}
 // Start of a proof (or of a host-driven call): the whole control block (state
 // and per-block count slots) to zero.
 __global__ void k_state_reset(unsigned long long* words, unsigned long long n_words) {
@@ -149,10 +152,19 @@ __global__ void k_init_hull(unsigned long long* p, unsigned long long n_records)
     }
 }
 
+// Serialises graph capture with device allocation and freeing: the runtime may
+// order those against the legacy stream, which a capture in progress on another
+// thread's stream does not allow.
+std::mutex g_capture_mutex;
+
+// Device bytes allocated by dalloc on this thread (a core records the bytes its
+// creation allocated).
+thread_local std::uint64_t t_dalloc_bytes = 0;
 template<typename T> T* dalloc(std::size_t n) {
     void* p = nullptr;
     if(n == 0) n = 1;
     hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    t_dalloc_bytes += n * sizeof(T);
     return static_cast<T*>(p);
 }
 
@@ -226,7 +238,12 @@ struct sqd_core {
     bool dirty = true;                // hull records and count slots need a full reset before use
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
-    bool fp32 = false;               // static levels run the FP32-prefiltered kernels
+    std::uint64_t device_bytes = 0;  // device memory allocated by sqd_core_create
+    bool fp32 = false;               // some static level runs the FP32-prefiltered kernels
+    // sqd_prove as HIP graphs: the initial queue with the first round, and one later round
+    hipGraphExec_t g_first = nullptr, g_round = nullptr;
+    unsigned proofs = 0;             // sqd_prove calls so far (graphs are captured from the second)
+    unsigned long long* d_range = nullptr;   // initial-queue range of the first variable's subdivisions
     unsigned long long* d_undec = nullptr;   // FP32 prefilter: undecided child indices of one level
     unsigned long long undec_cap = 0;
     unsigned long long k_max = 0;    // dynamic cuboids per round
@@ -323,21 +340,29 @@ void reset_state(sqd_core* c) {
         return hipGetLastError();
     });
 }
-// Folds the count slots into the counters and brings the device state to the
-// pinned host copy (one copy, one sync); checks the overflow flags.
-const DeviceState& read_state(sqd_core* c) {
+// Enqueues the fold of the count slots into the counters and the copy of the
+// device state to its pinned host copy.
+void enqueue_state_copy(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_fold_counts", [&] {
         hipLaunchKernelGGL(k_fold_counts, dim3(1), dim3(256), 0, c->stream, c->d_state, c->d_slots, c->n_slots);
         return hipGetLastError();
     });
     hip_check(hipMemcpyAsync(c->h_state, c->d_state, sizeof(DeviceState), hipMemcpyDeviceToHost, c->stream),
               "copy state");
+}
+// Waits for the state copy enqueued last and checks the overflow flags.
+const DeviceState& finish_state(sqd_core* c) {
     sync(c);
     if(c->h_state->counters.overflow)
         throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
     if(c->h_state->frontier_full)
         throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
     return *c->h_state;
+}
+// Folds the count slots and brings the device state to the host (one copy, one sync).
+const DeviceState& read_state(sqd_core* c) {
+    enqueue_state_copy(c);
+    return finish_state(c);
 }
 sqd_counts counts_of(const DeviceState& h) {
     sqd_counts r;
@@ -363,6 +388,7 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.undec = c->d_undec;
     a.undec_count = nullptr;
     a.undec_cap = c->undec_cap;
+    a.init_range = c->d_range;
     return a;
 }
 
@@ -387,7 +413,7 @@ void run_static(sqd_core* c, int mode, const dev::BoxList& in, const unsigned lo
         const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
         launch(c, mode * 16 + L, "static level launch",
                [&] {
-                   const int m = mode | (c->fp32 ? dev::kModeF32 : 0);
+                   const int m = mode | (((c->settings.fp32_prefilter >> L) & 1u) ? dev::kModeF32 : 0);
                    return c->ks->static_level(m, L, a, children, c->cus, c->grid_override, c->stream);
                });
         prev = out;
@@ -487,11 +513,44 @@ void launch_initial_box(sqd_core* c, bool to_scratch, bool reset) {
     });
 }
 
+// The range of the first dynamic variable's initial subdivisions that the next
+// initial queue starts from.
+void set_range(sqd_core* c, unsigned long long first, unsigned long long last) {
+    launch(c, KIND_SET_COUNT, "k_set_range", [&] {
+        hipLaunchKernelGGL(k_set_range, dim3(1), dim3(64), 0, c->stream, c->d_range, first, last);
+        return hipGetLastError();
+    });
+}
+unsigned long long full_range(const sqd_core* c) { return c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]; }
+
 // Enqueues a cleared device state and the initial queue on the (empty) stack;
 // nothing is waited for.
 void enqueue_initial_queue(sqd_core* c) {
     launch_initial_box(c, true, true);
     run_dynamic(c, dev::INIT, c->scratch.view(), c->scratch.count, 1, c->stack.view());
+}
+
+// Captures the launches and copies that `enqueue` makes on the core's stream
+// into an executable HIP graph (kernels and one device-to-host copy; nothing
+// that needs a memset node).
+template<typename F> hipGraphExec_t capture_graph(sqd_core* c, F&& enqueue) {
+    std::lock_guard<std::mutex> lock(g_capture_mutex);
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        enqueue();
+    } catch(...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(c->stream, &tmp);
+        if(tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(err, "hipGraphInstantiate");
+    return e;
 }
 
 // Enqueues one round over the top of the stack: pop (on the device), non-final
@@ -522,6 +581,10 @@ void enqueue_round(sqd_core* c) {
 } // namespace
 } // namespace sqd
 
+namespace sqd {
+void set_last_error(const std::string& message) { t_error = message; }
+}
+
 using namespace sqd;
 
 extern "C" {
@@ -532,6 +595,7 @@ void sqd_settings_default(sqd_settings* s) {
     if(!s) return;
     *s = sqd_settings{};
     s->generation_count = 8;
+    s->graphs = 1;
 }
 
 int sqd_lemma_count(void) { return static_cast<int>(sqd_lemmas::all_lemmas().size()); }
@@ -595,7 +659,13 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         if(!program || !out) throw std::invalid_argument("null argument");
         if(!program->ks)
             throw Failure(SQD_ERR_NO_KERNELS, "no kernels are compiled for this program (fingerprint not registered)");
-        auto c = std::make_unique<sqd_core>();
+        std::unique_lock<std::mutex> alloc_lock(g_capture_mutex);
+        struct Destroy {
+            std::unique_lock<std::mutex>* lock;
+            void operator()(sqd_core* p) const { lock->unlock(); sqd_core_destroy(p); }
+        };
+        std::unique_ptr<sqd_core, Destroy> c(new sqd_core, Destroy{&alloc_lock});   // frees what was allocated if creation fails
+        const std::uint64_t bytes0 = t_dalloc_bytes;
         c->device = device;
         c->program = program;
         c->ks = program->ks;
@@ -612,13 +682,16 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->cus = static_cast<unsigned>(prop.multiProcessorCount);
         c->grid_override = s.grid_blocks;
-        c->fp32 = s.fp32_prefilter != 0;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         const auto& prog = program->prog;
         for(const auto& sp : prog.splits) {
             if(sp.dynamic) { c->dyn_subs.push_back(sp.subs); c->dyn_initial.push_back(sp.initial); }
             else c->st_subs.push_back(sp.subs);
         }
+        unsigned used = 0;   // the levels the mask can select
+        for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L) used |= 1u << L;
+        c->fp32 = (s.fp32_prefilter & used) != 0;
+
         // level buffers hold the survivors of one level; size K for the worst case
         const unsigned long long per_entry = 16ull * static_cast<unsigned long long>(c->N) + 12ull;
         const unsigned long long lvl_cap = std::max<unsigned long long>(1024, s.level_bytes / per_entry);
@@ -653,35 +726,45 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->crit.create(c->N, s.critical_capacity, false, false, &st->crit_count);
         c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false, &st->scratch_count);
         c->d_hull = dalloc<unsigned long long>(static_cast<std::size_t>(kmax) * c->N * 4);
+        c->d_range = dalloc<unsigned long long>(2);
+        const unsigned long long full[2] = {0ull, c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]};
+        hip_check(hipMemcpyAsync(c->d_range, full, sizeof full, hipMemcpyHostToDevice, c->stream), "initial range");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         if(c->fp32) {   // as many undecided children as a level buffer holds boxes
             c->undec_cap = lvl_cap;
             c->d_undec = dalloc<unsigned long long>(static_cast<std::size_t>(lvl_cap));
         }
         c->d_has = dalloc<unsigned char>(static_cast<std::size_t>(kmax));
         c->d_consts = dalloc<double>(2 * static_cast<std::size_t>(std::max(1, c->ks->NC)));
-        hip_check(hipMemcpy(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
-                            hipMemcpyHostToDevice), "upload constants");
+        hip_check(hipMemcpyAsync(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
+                                 hipMemcpyHostToDevice, c->stream), "upload constants");
         c->d_box_ct = dalloc<double>(2 * static_cast<std::size_t>(c->N));
         c->d_box_init = dalloc<double>(2 * static_cast<std::size_t>(c->N));
         std::vector<double> box(prog.box_lb);
         box.insert(box.end(), prog.box_ub.begin(), prog.box_ub.end());
-        hip_check(hipMemcpy(c->d_box_ct, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice), "upload box");
+        hip_check(hipMemcpyAsync(c->d_box_ct, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                  "upload box");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->device_bytes = t_dalloc_bytes - bytes0;
         *out = c.release();
     });
 }
 
 void sqd_core_destroy(sqd_core* c) {
     if(!c) return;
+    std::lock_guard<std::mutex> lock(g_capture_mutex);
     (void)hipSetDevice(c->device);
     if(c->stream) (void)hipStreamSynchronize(c->stream);
     c->stack.destroy();
     for(auto& l : c->lvl) l.destroy();
     c->fin.destroy(); c->spl.destroy(); c->crit.destroy(); c->scratch.destroy();
     for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_state),
-                   static_cast<void*>(c->d_undec),
+                   static_cast<void*>(c->d_undec), static_cast<void*>(c->d_range),
                    static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
         if(p) (void)hipFree(p);
     if(c->h_state) (void)hipHostFree(c->h_state);
+    if(c->g_first) (void)hipGraphExecDestroy(c->g_first);
+    if(c->g_round) (void)hipGraphExecDestroy(c->g_round);
     for(hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -693,8 +776,9 @@ int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         launch_initial_box(c, false, false);
         sync(c);
-        hip_check(hipMemcpy(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
-        hip_check(hipMemcpy(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpyAsync(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream), "copy");
+        hip_check(hipMemcpyAsync(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream), "copy");
+        sync(c);
     });
 }
 
@@ -702,6 +786,7 @@ int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
+        set_range(c, 0, full_range(c));
         begin_call(c);
         enqueue_initial_queue(c);
         const DeviceState& h = read_state(c);
@@ -740,7 +825,8 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
             hlb.assign(static_cast<std::size_t>(c->N) * m, 0);
             hub.assign(static_cast<std::size_t>(c->N) * m, 0);
             download_soa(c, c->scratch, m, hlb.data(), hub.data(), m);
-            hip_check(hipMemcpy(has_hull + off, c->d_has, m, hipMemcpyDeviceToHost), "copy has");
+            hip_check(hipMemcpyAsync(has_hull + off, c->d_has, m, hipMemcpyDeviceToHost, c->stream), "copy has");
+            sync(c);
             for(int a = 0; a < c->N; ++a)
                 for(std::size_t i = 0; i < m; ++i) {
                     hull_lb[static_cast<std::size_t>(a) * n + off + i] = hlb[static_cast<std::size_t>(a) * m + i];
@@ -800,19 +886,47 @@ int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
 
 int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
               size_t capacity, size_t* n_critical, double* seconds) {
+    if(!c) { t_error = "null core"; return SQD_ERR_ARGUMENT; }
+    const uint32_t all = c->dyn_initial.empty() ? 1u : static_cast<uint32_t>(c->dyn_initial[0]);
+    return sqd_prove_range(c, 0, all, counts, crit_lb, crit_ub, capacity, n_critical, seconds);
+}
+
+int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* counts, double* crit_lb,
+                    double* crit_ub, size_t capacity, size_t* n_critical, double* seconds) {
     return guarded([&] {
         if(!c) throw std::invalid_argument("null core");
+        const uint32_t all = c->dyn_initial.empty() ? 1u : static_cast<uint32_t>(c->dyn_initial[0]);
+        if(first >= last || last > all)
+            throw std::invalid_argument("initial range must satisfy first < last <= the first variable's subdivisions");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         auto t0 = std::chrono::steady_clock::now();
+        set_range(c, first, last);
         begin_call(c);
-        enqueue_initial_queue(c);
         // One host round trip per round: the round's size is decided on the
-        // device, the host only learns whether the frontier is empty.
+        // device, the host only learns whether the frontier is empty.  From the
+        // second proof on a core (the first one fills the launch statistics),
+        // the rounds are replayed as captured graphs unless profiling is on.
+        const bool graphs = c->settings.graphs && !c->profiling && c->proofs > 0;
+        if(graphs && !c->g_first) {
+            c->g_first = capture_graph(c, [&] { enqueue_initial_queue(c); enqueue_round(c); enqueue_state_copy(c); });
+            c->g_round = capture_graph(c, [&] { enqueue_round(c); enqueue_state_copy(c); });
+        }
         const DeviceState* h = nullptr;
-        do {
-            enqueue_round(c);
-            h = &read_state(c);
-        } while(h->stack_count > 0);
+        if(graphs) {
+            hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
+            h = &finish_state(c);
+            while(h->stack_count > 0) {
+                hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
+                h = &finish_state(c);
+            }
+        } else {
+            enqueue_initial_queue(c);
+            do {
+                enqueue_round(c);
+                h = &read_state(c);
+            } while(h->stack_count > 0);
+        }
+        ++c->proofs;
         c->dirty = false;
         const sqd_counts r = counts_of(*h);
         const unsigned long long k = h->crit_count;
@@ -835,13 +949,29 @@ int sqd_core_criticals(sqd_core* c, double* crit_lb, double* crit_ub, size_t cap
     });
 }
 
+int sqd_core_shape(const sqd_core* c, uint32_t* initial_subdivisions, uint32_t* static_levels) {
+    return guarded([&] {
+        if(!c || !initial_subdivisions || !static_levels) throw std::invalid_argument("null argument");
+        *initial_subdivisions = static_cast<uint32_t>(full_range(c));
+        *static_levels = static_cast<uint32_t>(c->st_subs.size());
+    });
+}
+
+int sqd_core_memory(const sqd_core* c, uint64_t* device_bytes) {
+    return guarded([&] {
+        if(!c || !device_bytes) throw std::invalid_argument("null argument");
+        *device_bytes = c->device_bytes;
+    });
+}
+
 int sqd_core_acos_fallbacks(sqd_core* c, uint64_t* n) {
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         sync(c);
         unsigned long long v = 0;
-        hip_check(c->ks->acos_hard(&v), "acos counter");
+        hip_check(c->ks->acos_hard(&v, c->stream), "acos counter");
+        sync(c);
         *n = v;
     });
 }
@@ -868,17 +998,19 @@ int sqd_width_stats(sqd_core* c, const double* lb, const double* ub, size_t n, s
         const std::size_t box_bytes = sizeof(double) * static_cast<std::size_t>(c->N) * n;
         double *d_lb = nullptr, *d_ub = nullptr, *d_out = nullptr;
         std::vector<double> h(3 * static_cast<std::size_t>(nc), 0.0);
+        std::unique_lock<std::mutex> lock(g_capture_mutex);
         auto release = [&] { for(double* q : {d_lb, d_ub, d_out}) if(q) (void)hipFree(q); };
         try {
             d_lb = dalloc<double>(std::max<std::size_t>(1, static_cast<std::size_t>(c->N) * n));
             d_ub = dalloc<double>(std::max<std::size_t>(1, static_cast<std::size_t>(c->N) * n));
             d_out = dalloc<double>(3 * static_cast<std::size_t>(nc));
-            hip_check(hipMemcpy(d_lb, lb, box_bytes, hipMemcpyHostToDevice), "upload boxes");
-            hip_check(hipMemcpy(d_ub, ub, box_bytes, hipMemcpyHostToDevice), "upload boxes");
-            hip_check(hipMemcpy(d_out, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "clear");
+            hip_check(hipMemcpyAsync(d_lb, lb, box_bytes, hipMemcpyHostToDevice, c->stream), "upload boxes");
+            hip_check(hipMemcpyAsync(d_ub, ub, box_bytes, hipMemcpyHostToDevice, c->stream), "upload boxes");
+            hip_check(hipMemcpyAsync(d_out, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream), "clear");
             hip_check(c->ks->width_stats(d_lb, d_ub, n, c->d_consts, d_out, c->stream), "width statistics launch");
             hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-            hip_check(hipMemcpy(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost), "download");
+            hip_check(hipMemcpyAsync(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream), "download");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         } catch(...) { release(); throw; }
         release();
         for(int j = 0; j < nc && static_cast<size_t>(j) < capacity; ++j) {

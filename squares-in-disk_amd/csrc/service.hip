@@ -1,0 +1,448 @@
+// Proof service and autotuner, host code over the C ABI of the prover core.
+//
+// Service: proof jobs (a program plus a range of its first dynamic variable's
+// initial subdivisions, see sqd_prove_range) are queued as they arrive and run
+// by worker threads, each on a prover core of the job's program with its own
+// HIP stream, so that the kernels of several jobs overlap on the device (the
+// streams map onto the device's hardware queues) while each job's frontier stays
+// in HBM.  Cores are pooled per program and created on demand within a device
+// memory budget; an idle core of another program is destroyed to make room.  A
+// job is admitted when a worker and a core (or the memory for one) are free.
+//
+// Autotuner: times sqd_prove_range on a slice of the program's first variable
+// and picks, by coordinate descent, the FP32-prefilter level mask, the dynamic
+// cuboids per round (dequeue_buffer_size) and the blocks per level launch.
+// These settings change the schedule, not the counts (counts do not depend on
+// the order or the batching of the frontier); the prefilter could prune a child
+// binary64 keeps, so candidates whose counts differ from the plain run's are
+// rejected.
+//
+// Neither has a counterpart in the reference (its ProofDriver runs one proof
+// with one core per device, prover/proof_driver.hpp:393-438).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+#include "sqd.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t0, Clock::time_point t1) { return std::chrono::duration<double>(t1 - t0).count(); }
+
+}  // namespace
+
+namespace sqd {
+void set_last_error(const std::string& message);   // the message sqd_last_error() returns (runtime.hip)
+}
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+    sqd::set_last_error(msg);
+    return code;
+}
+
+// ---------------------------------------------------------------- autotuner
+struct Trial {
+    sqd_settings s;
+    double seconds = 0;
+    sqd_counts counts{};
+    int status = SQD_OK;
+};
+
+bool same_counts(const sqd_counts& a, const sqd_counts& b) {
+    return a.num_cuboids == b.num_cuboids && a.num_leaf_cuboids == b.num_leaf_cuboids &&
+           a.num_critical_cuboids == b.num_critical_cuboids;
+}
+
+// Median time of `reps` proofs of [lo, hi) on a fresh core with settings s
+// (one untimed proof first, which also captures the graphs).
+Trial time_settings(int device, const sqd_program* prog, const sqd_settings& s, uint32_t lo, uint32_t hi, int reps) {
+    Trial t;
+    t.s = s;
+    sqd_core* core = nullptr;
+    t.status = sqd_core_create(device, prog, &s, &core);
+    if(t.status != SQD_OK) return t;
+    std::vector<double> times;
+    for(int r = 0; r <= reps && t.status == SQD_OK; ++r) {
+        size_t nc = 0;
+        const auto t0 = Clock::now();
+        t.status = sqd_prove_range(core, lo, hi, &t.counts, nullptr, nullptr, 0, &nc, nullptr);
+        const auto t1 = Clock::now();
+        if(r > 0) times.push_back(seconds_since(t0, t1));
+    }
+    sqd_core_destroy(core);
+    if(t.status == SQD_OK) {
+        std::sort(times.begin(), times.end());
+        t.seconds = times[times.size() / 2];
+    }
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sqd_autotune_options_default(sqd_autotune_options* o) {
+    if(!o) return;
+    *o = sqd_autotune_options{};
+    o->target_seconds = 0.02;
+    o->repetitions = 3;
+}
+
+int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, const sqd_autotune_options* opt,
+                 sqd_settings* best, sqd_autotune_report* report) {
+    try {
+        if(!prog || !best) return fail(SQD_ERR_ARGUMENT, "null argument");
+        sqd_autotune_options o;
+        sqd_autotune_options_default(&o);
+        if(opt) o = *opt;
+        if(o.repetitions < 1) o.repetitions = 1;
+        sqd_settings s0;
+        if(base) s0 = *base;
+        else sqd_settings_default(&s0);
+        s0.fp32_prefilter = 0;
+        // the tuning workload: the whole range, or a centred slice of it whose
+        // proof takes about target_seconds
+        sqd_core* probe = nullptr;
+        int rc = sqd_core_create(device, prog, &s0, &probe);
+        if(rc != SQD_OK) return rc;
+        uint32_t all = 0, nstat = 0;
+        rc = sqd_core_shape(probe, &all, &nstat);
+        // start from the middle sixteenth, then halve or double the slice
+        uint32_t w = std::max<uint32_t>(1, all / 16);
+        auto centre = [&](uint32_t width, uint32_t& a, uint32_t& b) {
+            a = (all - width) / 2;
+            b = a + width;
+        };
+        uint32_t lo = 0, hi = all;
+        centre(w, lo, hi);
+        if(rc == SQD_OK) {
+            for(int step = 0; step < 12; ++step) {
+                sqd_counts c{};
+                size_t nc = 0;
+                const auto t0 = Clock::now();
+                rc = sqd_prove_range(probe, lo, hi, &c, nullptr, nullptr, 0, &nc, nullptr);
+                const double dt = seconds_since(t0, Clock::now());
+                if(rc != SQD_OK) break;
+                if(dt > o.target_seconds && w > 1) w = std::max<uint32_t>(1, w / 2);
+                else if(dt < o.target_seconds / 4 && w < all) w = std::min(all, w * 2);
+                else break;
+                centre(w, lo, hi);
+            }
+        }
+        unsigned cus = 0;
+        {
+            hipDeviceProp_t prop;
+            if(hipGetDeviceProperties(&prop, device) == hipSuccess) cus = static_cast<unsigned>(prop.multiProcessorCount);
+        }
+        sqd_core_destroy(probe);
+        if(rc != SQD_OK) return rc;
+
+        int trials = 0;
+        Trial plain = time_settings(device, prog, s0, lo, hi, o.repetitions);
+        ++trials;
+        if(plain.status != SQD_OK) return plain.status;
+        Trial cur = plain;
+        auto consider = [&](sqd_settings s) {
+            Trial t = time_settings(device, prog, s, lo, hi, o.repetitions);
+            ++trials;
+            if(t.status == SQD_OK && same_counts(t.counts, plain.counts) && t.seconds < cur.seconds) cur = t;
+        };
+        // 1. FP32 prefilter: every subset of the static levels
+        const uint32_t masks = nstat >= 8 ? 256u : (1u << nstat);
+        const sqd_settings s_mask = cur.s;
+        for(uint32_t m = 1; m < masks; ++m) {
+            sqd_settings s = s_mask;
+            s.fp32_prefilter = m;
+            consider(s);
+        }
+        // 2. dynamic cuboids per round
+        const sqd_settings s_k = cur.s;
+        for(uint32_t k : {1024u, 4096u, 16384u, 65536u}) {
+            sqd_settings s = s_k;
+            s.dequeue_buffer_size = k;
+            consider(s);
+        }
+        // 3. blocks per level launch (0: the kernel's resident capacity)
+        const sqd_settings s_g = cur.s;
+        for(uint32_t g : {cus, 2 * cus, 4 * cus}) {
+            if(g == 0) continue;
+            sqd_settings s = s_g;
+            s.grid_blocks = g;
+            consider(s);
+        }
+        *best = cur.s;
+        if(report) {
+            report->first = lo;
+            report->last = hi;
+            report->trials = static_cast<uint32_t>(trials);
+            report->plain_seconds = plain.seconds;
+            report->best_seconds = cur.seconds;
+            report->counts = cur.counts;
+        }
+        return SQD_OK;
+    } catch(const std::exception& e) {
+        return fail(SQD_ERR_INTERNAL, e.what());
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- service
+struct sqd_service {
+    sqd_service_options opt{};
+    std::mutex m;
+    std::condition_variable cv;        // queue, cores and job state changes
+    struct Job {
+        const sqd_program* prog = nullptr;
+        uint32_t first = 0, last = 0;
+        Clock::time_point submitted, started, finished;
+        bool done = false;
+        sqd_job_result result{};
+        std::string error;
+    };
+    std::map<uint64_t, Job> jobs;
+    std::deque<uint64_t> queue;
+    uint64_t next_id = 1;
+    struct Slot {
+        uint64_t id;
+        const sqd_program* prog;
+        sqd_core* core;       // nullptr while the worker that reserved the slot creates it
+        uint64_t bytes;
+        bool busy;
+    };
+    std::vector<Slot> pool;
+    uint64_t next_slot = 1;
+    uint64_t bytes_in_use = 0;
+    std::map<const sqd_program*, uint64_t> core_bytes;     // measured size of a core per program
+    std::map<const sqd_program*, sqd_settings> settings;   // per program (autotuned or the defaults)
+    std::map<const sqd_program*, bool> tuning;             // autotune in progress
+    bool stop = false;
+    std::vector<std::thread> workers;
+
+    sqd_settings base_settings() const {
+        sqd_settings s;
+        sqd_settings_default(&s);
+        s.level_bytes = opt.level_bytes;
+        s.frontier_capacity = opt.frontier_capacity;
+        s.fp32_prefilter = opt.fp32_prefilter;
+        return s;
+    }
+
+    Slot* find_slot(uint64_t id) {
+        for(auto& s : pool)
+            if(s.id == id) return &s;
+        return nullptr;
+    }
+
+    // With the lock held: reserves a free core of `prog` (core set) or room for
+    // a new one within the memory budget (core == nullptr: the caller creates
+    // it), destroying idle cores of other programs or waiting as needed.
+    // Returns the slot id, 0 when the service stops.
+    uint64_t acquire(std::unique_lock<std::mutex>& lk, const sqd_program* prog) {
+        for(;;) {
+            if(stop) return 0;
+            for(auto& s : pool)
+                if(s.prog == prog && !s.busy && s.core) {
+                    s.busy = true;
+                    return s.id;
+                }
+            const uint64_t need = core_bytes.count(prog) ? core_bytes[prog] : opt.core_bytes_estimate;
+            if(bytes_in_use + need <= opt.memory_budget || pool.empty()) {
+                pool.push_back(Slot{next_slot++, prog, nullptr, need, true});
+                bytes_in_use += need;
+                return pool.back().id;
+            }
+            // make room: destroy an idle core of another program
+            bool freed = false;
+            for(size_t i = 0; i < pool.size(); ++i)
+                if(!pool[i].busy && pool[i].core) {
+                    sqd_core* c = pool[i].core;
+                    bytes_in_use -= pool[i].bytes;
+                    pool.erase(pool.begin() + static_cast<std::ptrdiff_t>(i));
+                    lk.unlock();
+                    sqd_core_destroy(c);
+                    lk.lock();
+                    freed = true;
+                    break;
+                }
+            if(!freed) cv.wait(lk);
+        }
+    }
+
+    void run_worker(int device) {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(m);
+        for(;;) {
+            cv.wait(lk, [&] { return stop || !queue.empty(); });
+            if(stop) return;
+            const uint64_t id = queue.front();
+            queue.pop_front();
+            const sqd_program* prog = jobs[id].prog;
+            // settings of the program (tuned once, by the first worker that needs them)
+            if(!settings.count(prog)) {
+                if(opt.autotune && !tuning[prog]) {
+                    tuning[prog] = true;
+                    lk.unlock();
+                    sqd_settings best = base_settings(), base = base_settings();
+                    if(sqd_autotune(device, prog, &base, nullptr, &best, nullptr) != SQD_OK) best = base_settings();
+                    best.level_bytes = opt.level_bytes;
+                    best.frontier_capacity = opt.frontier_capacity;
+                    lk.lock();
+                    settings[prog] = best;
+                    tuning[prog] = false;
+                    cv.notify_all();
+                } else if(opt.autotune) {
+                    cv.wait(lk, [&] { return stop || settings.count(prog) != 0; });
+                    if(stop) return;
+                } else {
+                    settings[prog] = base_settings();
+                }
+            }
+            const uint64_t slot = acquire(lk, prog);
+            if(slot == 0) return;
+            jobs[id].started = Clock::now();
+            const uint32_t first = jobs[id].first, last = jobs[id].last;
+            const sqd_settings s = settings[prog];
+            sqd_core* core = find_slot(slot)->core;
+            lk.unlock();
+            int rc = SQD_OK;
+            std::string err;
+            if(!core) {
+                rc = sqd_core_create(device, prog, &s, &core);
+                if(rc != SQD_OK) err = sqd_last_error();
+            }
+            sqd_job_result r{};
+            if(rc == SQD_OK) {
+                size_t nc = 0;
+                rc = sqd_prove_range(core, first, last, &r.counts, nullptr, nullptr, 0, &nc, nullptr);
+                if(rc != SQD_OK) err = sqd_last_error();
+                r.n_critical = nc;
+            }
+            uint64_t bytes = 0;
+            if(core) sqd_core_memory(core, &bytes);
+            lk.lock();
+            Slot* sl = find_slot(slot);          // busy slots are never removed by others
+            if(core) {
+                if(bytes) {
+                    bytes_in_use = bytes_in_use - sl->bytes + bytes;
+                    sl->bytes = bytes;
+                    core_bytes[prog] = bytes;
+                }
+                sl->core = core;
+                sl->busy = false;
+            } else {                              // creation failed: give the room back
+                bytes_in_use -= sl->bytes;
+                pool.erase(pool.begin() + (sl - pool.data()));
+            }
+            Job& j = jobs[id];
+            j.finished = Clock::now();
+            r.status = rc;
+            r.queued_seconds = seconds_since(j.submitted, j.started);
+            r.run_seconds = seconds_since(j.started, j.finished);
+            r.latency_seconds = seconds_since(j.submitted, j.finished);
+            j.result = r;
+            j.error = err;
+            j.done = true;
+            cv.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+void sqd_service_options_default(sqd_service_options* o) {
+    if(!o) return;
+    *o = sqd_service_options{};
+    o->device = 0;
+    o->workers = 8;
+    o->memory_budget = 64ull << 30;
+    o->fp32_prefilter = SQD_FP32_ALL_LEVELS;
+    o->autotune = 0;
+    o->level_bytes = 256ull << 20;
+    o->frontier_capacity = 1ull << 22;
+    o->core_bytes_estimate = 2ull << 30;
+}
+
+int sqd_service_create(const sqd_service_options* opt, sqd_service** out) {
+    try {
+        if(!out) return fail(SQD_ERR_ARGUMENT, "null argument");
+        auto* s = new sqd_service;
+        sqd_service_options_default(&s->opt);
+        if(opt) s->opt = *opt;
+        if(s->opt.workers < 1) s->opt.workers = 1;
+        if(s->opt.workers > 64) s->opt.workers = 64;
+        if(s->opt.core_bytes_estimate == 0) s->opt.core_bytes_estimate = 2ull << 30;
+        const int device = s->opt.device;
+        for(uint32_t i = 0; i < s->opt.workers; ++i) s->workers.emplace_back([s, device] { s->run_worker(device); });
+        *out = s;
+        return SQD_OK;
+    } catch(const std::exception& e) {
+        return fail(SQD_ERR_INTERNAL, e.what());
+    }
+}
+
+int sqd_service_submit(sqd_service* s, const sqd_program* prog, uint32_t first, uint32_t last, uint64_t* job) {
+    if(!s || !prog || !job) return fail(SQD_ERR_ARGUMENT, "null argument");
+    if(first >= last) return fail(SQD_ERR_ARGUMENT, "empty initial range");
+    std::lock_guard<std::mutex> lk(s->m);
+    const uint64_t id = s->next_id++;
+    sqd_service::Job& j = s->jobs[id];
+    j.prog = prog;
+    j.first = first;
+    j.last = last;
+    j.submitted = Clock::now();
+    s->queue.push_back(id);
+    *job = id;
+    s->cv.notify_all();
+    return SQD_OK;
+}
+
+int sqd_service_poll(sqd_service* s, uint64_t job, int* done, sqd_job_result* out) {
+    if(!s || !done) return fail(SQD_ERR_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->jobs.find(job);
+    if(it == s->jobs.end()) return fail(SQD_ERR_ARGUMENT, "unknown job");
+    *done = it->second.done ? 1 : 0;
+    if(it->second.done && out) *out = it->second.result;
+    return SQD_OK;
+}
+
+int sqd_service_wait(sqd_service* s, uint64_t job, sqd_job_result* out) {
+    if(!s) return fail(SQD_ERR_ARGUMENT, "null argument");
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->jobs.find(job);
+    if(it == s->jobs.end()) return fail(SQD_ERR_ARGUMENT, "unknown job");
+    s->cv.wait(lk, [&] { return it->second.done || s->stop; });
+    if(!it->second.done) return fail(SQD_ERR_INTERNAL, "service stopped");
+    if(out) *out = it->second.result;
+    const sqd_service::Job j = it->second;
+    s->jobs.erase(it);
+    if(j.result.status != SQD_OK) sqd::set_last_error(j.error);
+    return j.result.status;
+}
+
+void sqd_service_destroy(sqd_service* s) {
+    if(!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    for(auto& t : s->workers) t.join();
+    for(auto& slot : s->pool)
+        if(slot.core) sqd_core_destroy(slot.core);
+    delete s;
+}
+
+}  // extern "C"

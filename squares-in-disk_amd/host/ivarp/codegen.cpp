@@ -5,6 +5,7 @@
 // dropped from the generated code.
 #include "codegen.hpp"
 #include "fixed_point.hpp"
+#include <algorithm>
 #include <sstream>
 #include <stdexcept>
 #include <map>
@@ -262,16 +263,17 @@ std::string generate(const flat::Program& p, const std::string& key) {
         }
     }
     body << "        default: break;\n        }\n    }\n";
-    // constraint rows per static level
+    // constraint rows per static level, tested in row order (cpu_prover_core.hpp:179-191)
     body << "    template<int L, class Num> __device__ static bool rows_violated(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
     body << "        (void)lb; (void)ub; (void)C;\n";
     for(std::size_t L = 0; L < sta.size(); ++L) {
         int from = p.splits[static_cast<std::size_t>(sta[L])].arg;
         int to = L + 1 < sta.size() ? p.splits[static_cast<std::size_t>(sta[L + 1])].arg : p.nargs;
-        body << "        if constexpr(L == " << L << ") {\n";
+        std::vector<int> level;
         for(int r = from; r < to; ++r)
-            for(int c : p.rows[static_cast<std::size_t>(r)])
-                body << "            if(!c" << c << "(lb, ub, C).poss) return true;\n";
+            for(int c : p.rows[static_cast<std::size_t>(r)]) level.push_back(c);
+        body << "        if constexpr(L == " << L << ") {\n";
+        for(int c : level) body << "            if(!c" << c << "(lb, ub, C).poss) return true;\n";
         body << "        }\n";
     }
     body << "        return false;\n    }\n";

@@ -37,12 +37,42 @@ class Settings(ctypes.Structure):
     _fields_ = [("generation_count", ctypes.c_uint32), ("dequeue_buffer_size", ctypes.c_uint32),
                 ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
                 ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
-                ("fp32_prefilter", ctypes.c_uint32)]
+                ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+FP32_ALL_LEVELS = 0xFFFFFFFF
 
 
 class WidthStat(ctypes.Structure):
     _fields_ = [("constraint", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("samples", ctypes.c_uint64),
                 ("mean_ratio", ctypes.c_double), ("max_ratio", ctypes.c_double)]
+
+
+class AutotuneOptions(ctypes.Structure):
+    _fields_ = [("target_seconds", ctypes.c_double), ("repetitions", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+
+
+class AutotuneReport(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_uint32), ("last", ctypes.c_uint32), ("trials", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("plain_seconds", ctypes.c_double), ("best_seconds", ctypes.c_double),
+                ("counts", Counts)]
+
+
+class ServiceOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("workers", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64),
+                ("fp32_prefilter", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("level_bytes", ctypes.c_uint64),
+                ("frontier_capacity", ctypes.c_uint64), ("core_bytes_estimate", ctypes.c_uint64)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("counts", Counts), ("n_critical", ctypes.c_uint64), ("queued_seconds", ctypes.c_double),
+                ("run_seconds", ctypes.c_double), ("latency_seconds", ctypes.c_double), ("status", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {"counts": self.counts.as_dict(), "n_critical": int(self.n_critical),
+                "queued_seconds": self.queued_seconds, "run_seconds": self.run_seconds,
+                "latency_seconds": self.latency_seconds, "status": int(self.status)}
 
 
 class KernelStat(ctypes.Structure):
@@ -87,6 +117,19 @@ def lib():
             "sqd_prove": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _D, _D, _SZ, ctypes.POINTER(_SZ),
                                          ctypes.POINTER(ctypes.c_double)]),
             "sqd_core_criticals": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(_SZ)]),
+            "sqd_prove_range": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Counts), _D, _D, _SZ,
+                                               ctypes.POINTER(_SZ), ctypes.POINTER(ctypes.c_double)]),
+            "sqd_core_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+            "sqd_core_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_autotune_options_default": (None, [ctypes.POINTER(AutotuneOptions)]),
+            "sqd_autotune": (ctypes.c_int, [ctypes.c_int, _P, ctypes.POINTER(Settings), ctypes.POINTER(AutotuneOptions),
+                                            ctypes.POINTER(Settings), ctypes.POINTER(AutotuneReport)]),
+            "sqd_service_options_default": (None, [ctypes.POINTER(ServiceOptions)]),
+            "sqd_service_create": (ctypes.c_int, [ctypes.POINTER(ServiceOptions), ctypes.POINTER(_P)]),
+            "sqd_service_submit": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_service_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(JobResult)]),
+            "sqd_service_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
+            "sqd_service_destroy": (None, [_P]),
             "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
@@ -110,7 +153,10 @@ def exported_symbols():
     return ["sqd_settings_default", "sqd_program_from_text", "sqd_program_from_lemma", "sqd_program_destroy",
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
-            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_core_criticals",
+            "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_prove_range",
+            "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
+            "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
+            "sqd_service_wait", "sqd_service_destroy",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
             "sqd_rounded_op", "sqd_interval_op",
@@ -273,6 +319,26 @@ class Core:
                 "seconds": secs.value}
 
 
+    def prove_range(self, first, last):
+        """Proof restricted to the first variable's initial subdivisions [first, last)."""
+        c = Counts()
+        k = _SZ()
+        secs = ctypes.c_double()
+        _check(lib().sqd_prove_range(self._h, first, last, ctypes.byref(c), None, None, 0, ctypes.byref(k),
+                                     ctypes.byref(secs)))
+        return {"counts": c.as_dict(), "n_critical": int(k.value), "seconds": secs.value}
+
+    def shape(self):
+        """(initial subdivisions of the first dynamic variable, static levels)."""
+        a, b = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().sqd_core_shape(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
+
+    def memory(self):
+        n = ctypes.c_uint64()
+        _check(lib().sqd_core_memory(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def acos_fallbacks(self):
         """Undecided acos roundings of this core's program (results widened by one double)."""
         n = ctypes.c_uint64()
@@ -346,3 +412,58 @@ def interval_op(op, a, b=None, device=0):
     out = np.zeros_like(a)
     _check(lib().sqd_interval_op(device, INTERVAL_OPS[op], a.shape[0], _dptr(a), _dptr(b), _dptr(out)))
     return out
+
+
+def autotune(program, device=0, base=None, target_seconds=0.02, repetitions=3):
+    """sqd_autotune: the best settings found and the report (dict)."""
+    o = AutotuneOptions()
+    lib().sqd_autotune_options_default(ctypes.byref(o))
+    o.target_seconds = target_seconds
+    o.repetitions = repetitions
+    best = Settings()
+    rep = AutotuneReport()
+    _check(lib().sqd_autotune(device, program._h, ctypes.byref(base) if base is not None else None, ctypes.byref(o),
+                              ctypes.byref(best), ctypes.byref(rep)))
+    return best, {"first": rep.first, "last": rep.last, "trials": rep.trials, "plain_seconds": rep.plain_seconds,
+                  "best_seconds": rep.best_seconds, "counts": rep.counts.as_dict(),
+                  "settings": {f: getattr(best, f) for f, _ in Settings._fields_}}
+
+
+class Service:
+    """The proof service (sqd_service): submit jobs, wait for their results."""
+
+    def __init__(self, device=0, **options):
+        o = ServiceOptions()
+        lib().sqd_service_options_default(ctypes.byref(o))
+        o.device = device
+        for k, v in options.items():
+            setattr(o, k, v)
+        self._h = _P()
+        _check(lib().sqd_service_create(ctypes.byref(o), ctypes.byref(self._h)))
+
+    def submit(self, program, first, last):
+        j = ctypes.c_uint64()
+        _check(lib().sqd_service_submit(self._h, program._h, first, last, ctypes.byref(j)))
+        return int(j.value)
+
+    def poll(self, job):
+        done = ctypes.c_int()
+        r = JobResult()
+        _check(lib().sqd_service_poll(self._h, job, ctypes.byref(done), ctypes.byref(r)))
+        return r.as_dict() if done.value else None
+
+    def wait(self, job):
+        r = JobResult()
+        _check(lib().sqd_service_wait(self._h, job, ctypes.byref(r)))
+        return r.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().sqd_service_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -29,7 +29,7 @@ INCLUDES = [ROOT / "include", PKG / "csrc", PKG / "host", PKG / "host" / "ivarp"
 HOST_SOURCES = [PKG / "host/ivarp/expr.cpp", PKG / "host/ivarp/lower.cpp", PKG / "host/ivarp/program.cpp",
                 PKG / "lemmas/lemmas.cpp"]
 TOOL_SOURCES = HOST_SOURCES + [PKG / "host/ivarp/codegen.cpp", PKG / "host/tools/sqd_lower.cpp"]
-HIP_SOURCES = [PKG / "csrc/runtime.hip", PKG / "csrc/arith_test.hip"]
+HIP_SOURCES = [PKG / "csrc/runtime.hip", PKG / "csrc/service.hip", PKG / "csrc/arith_test.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall"]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-std=c++17", "-O3", "-fPIC", "-ffp-contract=off",

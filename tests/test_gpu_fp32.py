@@ -55,7 +55,7 @@ def fp32_cores():
         k = (key, generations)
         if k not in made:
             p = sqd.Program.from_lemma(key)
-            s = sqd.default_settings(generation_count=generations, fp32_prefilter=1)
+            s = sqd.default_settings(generation_count=generations, fp32_prefilter=sqd.FP32_ALL_LEVELS)
             made[k] = (p, sqd.Core(p, settings=s))
         return made[k][1]
     yield get

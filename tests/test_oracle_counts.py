@@ -103,3 +103,16 @@ def test_oracle_fp32_prefilter_keeps_published_counts(program_text, key):
     assert counts["num_critical_cuboids"] == 0
     checked, decided = o.fp32_stats()
     assert checked > 0 and 0.95 < decided / checked <= 1.0
+
+
+def test_oracle_init_range_partition_adds_up(program_text):
+    # a job's range of the first variable's initial subdivisions is an exact
+    # union of subtrees: a partition of [0, 128) adds up to the published totals
+    tot = [0, 0]
+    for a, b in [(0, 40), (40, 64), (64, 100), (100, 128)]:
+        o = pyoracle.Oracle(program_text("one"))
+        o.set_init_range(a, b)
+        c, _ = o.run(threads=8)
+        tot[0] += c["num_cuboids"]
+        tot[1] += c["num_leaf_cuboids"]
+    assert tuple(tot) == PUBLISHED["one"]
