@@ -183,7 +183,7 @@ def initial_subdivisions(program):
 def fp32_first_batched(sqd, prog, expected, workers, proofs, plain_value, counted32):
     """The batched FP32-first leg on `prog` (full proofs through the service)."""
     best, tune = sqd.autotune(prog)
-    tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "dequeue_buffer_size", "grid_blocks")}
+    tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "prefilter_group", "dequeue_buffer_size", "grid_blocks")}
     n = initial_subdivisions(prog)
     # decided fraction and the dominant prefiltered kernel's time on one core
     with sqd.Core(prog, settings=best) as core:

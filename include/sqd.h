@@ -68,7 +68,9 @@ typedef struct sqd_settings {
                                       everything in binary64) */
     uint32_t graphs;               /* nonzero: sqd_prove replays its rounds as captured HIP
                                       graphs from the second proof on a core (default 1) */
-    uint32_t reserved;
+    uint32_t prefilter_group;      /* FP32 prefilter: most children of one parent a lane
+                                      evaluates together, sharing the terms free of the
+                                      split variable (1, 2, 4, 8; 0 = chosen per launch) */
 } sqd_settings;
 
 #define SQD_FP32_ALL_LEVELS 0xffffffffu
@@ -190,14 +192,15 @@ int sqd_core_shape(const sqd_core* core, uint32_t* initial_subdivisions, uint32_
 /* ---------------------------------------------------------------- autotuner */
 
 /* Chooses, by coordinate descent over timed sqd_prove_range runs on a slice of
- * the program's first variable, the FP32-prefilter level mask, the dynamic
- * cuboids per round (dequeue_buffer_size) and the blocks per level launch
- * (grid_blocks), starting from `base` (NULL: the defaults).  Candidates whose
+ * the program's first variable, the FP32-prefilter level mask and children per
+ * lane (prefilter_group), the dynamic cuboids per round (dequeue_buffer_size)
+ * and the blocks per level launch (grid_blocks), starting from `base` (NULL: the defaults).  Candidates whose
  * counts differ from the plain binary64 run are rejected.  No counterpart in the
  * reference. */
 typedef struct sqd_autotune_options {
-    double target_seconds;    /* slice of the first variable to time: about this long (default 0.02) */
-    int32_t repetitions;      /* timed proofs per candidate, median taken (default 3) */
+    double target_seconds;    /* slice of the first variable to time: about this long, and about
+                                 this much timing per candidate (default 0.02) */
+    int32_t repetitions;      /* at least this many timed proofs per candidate, median taken (default 3) */
     uint32_t reserved;
 } sqd_autotune_options;
 typedef struct sqd_autotune_report {

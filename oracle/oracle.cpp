@@ -652,13 +652,22 @@ struct Core {
             for(int cn : p.rows[static_cast<std::size_t>(r)]) if(!ev.possibly(cn, c)) return true;
         return false;
     }
-    // FP32 prefilter of one static-level child: true when binary32 interval
-    // arithmetic on the outward-converted box proves it empty or infeasible.
+    // FP32 prefilter of one static-level child q (the parent box with the split
+    // variable `arg` replaced by the child's subrange), in binary32 on the
+    // outward-converted box: the child is pruned when propagation empties it, or
+    // when a constraint of its rows fails on the hybrid box made of the parent's
+    // other variables and the propagated child's split variable.  The hybrid box
+    // contains the propagated child (propagation only narrows), so both tests
+    // are rigorous; the hybrid box makes every term free of `arg` equal for all
+    // children of a parent.
     bool f32_prunes(Evaluator32& ev32, const Cuboid& q, int arg, int row_end) const {
-        CuboidT<float> f;
+        CuboidT<float> f, h;
         for(std::size_t a = 0; a < q.lb.size(); ++a) { f.lb.push_back(f32_down(q.lb[a])); f.ub.push_back(f32_up(q.ub[a])); }
+        h = f;
         if(propagate(ev32, f, arg, static_cast<std::size_t>(p.N), 4)) return true;
-        return violates_rows(ev32, f, arg, row_end);
+        h.lb[static_cast<std::size_t>(arg)] = f.lb[static_cast<std::size_t>(arg)];
+        h.ub[static_cast<std::size_t>(arg)] = f.ub[static_cast<std::size_t>(arg)];
+        return violates_rows(ev32, h, arg, row_end);
     }
     // Static enumeration (cpu_prover_core.hpp:221-256); `leaf` is called for each
     // surviving leaf.  With ev32 set, each child is first tried in binary32

@@ -27,6 +27,7 @@
 //   rows_violated<L>(lb, ub, C)   constraint rows of static level L
 //   all_satisfied(lb, ub, C)      every remaining constraint possibly holds
 #pragma once
+#include <type_traits>
 #include "interval.hpp"
 
 namespace sqd {
@@ -46,6 +47,8 @@ constexpr int kBlock = 256;
 #endif
 
 enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
+// Hoisted-term struct of a program without static levels.
+struct NoHoist {};
 // Flag or'ed into a static-level mode at launch: the FP32-prefiltered kernel.
 constexpr int kModeF32 = 4;
 
@@ -88,6 +91,8 @@ struct LevelArgs {
     // initial queue: the first dynamic variable's initial subdivisions [range[0], range[1])
     // (nullptr: all of them)
     const unsigned long long* init_range;
+    // FP32 prefilter: most children of one parent per lane (0: chosen from the launch size)
+    int children_per_lane;
 };
 
 // ---------------------------------------------------------------- hull keys
@@ -320,6 +325,12 @@ __device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, u
 }
 
 // ---------------------------------------------------------------- static levels
+// Most children of one parent that one lane of the FP32 prefilter handles (the
+// terms free of the level's variable are evaluated once for them).
+template<int SUBS> constexpr int children_per_lane() {
+    return SUBS % 8 == 0 ? 8 : (SUBS % 4 == 0 ? 4 : (SUBS % 2 == 0 ? 2 : 1));
+}
+
 // Child w of a static level: the parent box with the level's variable replaced
 // by subrange w % SUBS of the splitter (splitter.hpp:44-59), its dynamic cuboid
 // (origin) and its enumeration index within that cuboid's grid (leaf).
@@ -417,20 +428,27 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
     block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
 }
 
-// The FP32 prefilter of static level L: every child is propagated and checked
-// in binary32 on its outward-rounded box with outward binary32 constants.  A
-// child that binary32 proves empty or infeasible is a pruned leaf (a rigorous
-// verdict: the binary32 intervals enclose the real ranges); the others are
-// appended to args.undec for k_static_level<..., true>, which decides them in
-// binary64.  Survivors, hulls and criticals therefore always come from the
-// binary64 evaluation; only the set of pruned leaves may differ from the plain
-// pass.  The binary32 pass runs in its own kernel so that its register budget
-// (and with it the occupancy) is not that of the binary64 code.
+// The FP32 prefilter of static level L.  Each child (the parent box with the
+// level's variable A split) is tested in binary32 on outward-rounded boxes with
+// outward binary32 constants: it is pruned when propagation empties it, or when
+// a constraint of its rows fails on the hybrid box made of the parent's other
+// variables and the propagated child's A.  Both tests are rigorous (the hybrid
+// box contains the propagated child, since propagation only narrows).  The
+// other children are appended to args.undec for k_static_level<..., true>,
+// which decides them in binary64; survivors, hulls and criticals therefore
+// always come from the binary64 evaluation.
+//
+// On the hybrid box every term free of A is the same for all children of a
+// parent, so each lane takes up to CPL children of one parent and evaluates those
+// terms once (P::hoist), then the rest per child (P::rows_violated_hoisted).
+// The binary32 pass runs in its own kernel so that its register budget (and
+// with it the occupancy) is not that of the binary64 code.
 template<class P, int MODE, int L>
 __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(LevelArgs args) {
     constexpr int N = P::N;
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
+    constexpr int CPL = children_per_lane<SUBS>();   // most children per lane
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
     __shared__ float s_c32[2 * (P::NC > 0 ? P::NC : 1)];
     __shared__ PropScratch<P> s_prop;
@@ -439,34 +457,75 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
-    const unsigned long long total = n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    // children per lane: the most that still leave two lane slots per thread of
+    // the grid (small launches need the parallelism more than the hoisting)
+    int cpl = 1;
+    for(int c = CPL; c > 1; c >>= 1) {
+        if(SUBS % c != 0) continue;
+        if(args.children_per_lane > 0 ? c <= args.children_per_lane
+                                      : n_items * static_cast<unsigned long long>(SUBS / c) >= 2 * stride) {
+            cpl = c;
+            break;
+        }
+    }
+    const int step = SUBS / cpl;                                                       // lanes per parent
+    const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // lane slots
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const unsigned long long trips = (total + stride - 1) / stride;
     unsigned long long c_cub = 0, c_leaf = 0, c_dec = 0, c_chk = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
-        const unsigned long long w = first + trip * stride;
-        bool undecided = false;
-        if(w < total) {
-            if(w % SUBS == 0) c_cub += SUBS;
+        const unsigned long long slot = first + trip * stride;
+        const bool valid = slot < total;
+        const unsigned long long item = slot / static_cast<unsigned long long>(step);
+        const int j = static_cast<int>(slot % static_cast<unsigned long long>(step));
+        float plb[N], pub[N];
+        double pl = 0, pu = 0, wd = 0;
+        typename P::template Hoist<L, float> h;
+        if(valid) {
             double lb[N], ub[N];
-            unsigned origin, leaf;
-            static_child<P, L>(args, in_base, w, lb, ub, origin, leaf);
-            float lb32[N], ub32[N];
+            load_box<P>(args.in, in_base + item, lb, ub);
 #pragma unroll
-            for(int a = 0; a < N; ++a) { lb32[a] = f32_down(lb[a]); ub32[a] = f32_up(ub[a]); }
-            ++c_chk;
-            if(propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop) || P::template rows_violated<L>(lb32, ub32, s_c32)) {
-                ++c_leaf;
-                ++c_dec;
-            } else {
-                undecided = true;
-            }
+            for(int a = 0; a < N; ++a) { plb[a] = f32_down(lb[a]); pub[a] = f32_up(ub[a]); }
+            pl = lb[ARG];
+            pu = ub[ARG];
+            wd = split_width(pl, pu, SUBS);
+            P::template hoist<L>(plb, pub, s_c32, h);
         }
-        const unsigned long long pos = wave_append(undecided, args.undec_count);
-        if(undecided) {
-            if(pos < args.undec_cap) args.undec[pos] = w;
-            else atomicAdd(&args.counters->overflow, 1ull);
+#pragma unroll 1
+        for(int k = 0; k < cpl; ++k) {
+            const int i = j + k * step;
+            bool undecided = false;
+            if(valid) {
+                if(i == 0) c_cub += SUBS;
+                float lb32[N], ub32[N];
+#pragma unroll
+                for(int a = 0; a < N; ++a) { lb32[a] = plb[a]; ub32[a] = pub[a]; }
+                lb32[ARG] = f32_down(split_point(pl, pu, wd, i, SUBS));
+                ub32[ARG] = f32_up(split_point(pl, pu, wd, i + 1, SUBS));
+                ++c_chk;
+                bool pruned = propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop);
+                if(!pruned) {
+                    float hl[N], hu[N];
+#pragma unroll
+                    for(int a = 0; a < N; ++a) { hl[a] = plb[a]; hu[a] = pub[a]; }
+                    hl[ARG] = lb32[ARG];
+                    hu[ARG] = ub32[ARG];
+                    pruned = P::template rows_violated_hoisted<L>(hl, hu, s_c32, h);
+                }
+                if(pruned) {
+                    ++c_leaf;
+                    ++c_dec;
+                } else {
+                    undecided = true;
+                }
+            }
+            const unsigned long long w = item * SUBS + static_cast<unsigned long long>(i);
+            const unsigned long long pos = wave_append(undecided, args.undec_count);
+            if(undecided) {
+                if(pos < args.undec_cap) args.undec[pos] = w;
+                else atomicAdd(&args.counters->overflow, 1ull);
+            }
         }
     }
     block_counts_add(args.block_counts, c_cub, c_leaf, 0ull, c_dec, c_chk);

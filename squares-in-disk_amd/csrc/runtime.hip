@@ -389,6 +389,7 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.undec_count = nullptr;
     a.undec_cap = c->undec_cap;
     a.init_range = c->d_range;
+    a.children_per_lane = static_cast<int>(std::min<uint32_t>(c->settings.prefilter_group, 64u));
     return a;
 }
 

@@ -10,8 +10,9 @@
 // job is admitted when a worker and a core (or the memory for one) are free.
 //
 // Autotuner: times sqd_prove_range on a slice of the program's first variable
-// and picks, by coordinate descent, the FP32-prefilter level mask, the dynamic
-// cuboids per round (dequeue_buffer_size) and the blocks per level launch.
+// and picks, by coordinate descent, the FP32-prefilter level mask, its children
+// per lane, the dynamic cuboids per round (dequeue_buffer_size) and the blocks
+// per level launch.
 // These settings change the schedule, not the counts (counts do not depend on
 // the order or the batching of the frontier); the prefilter could prune a child
 // binary64 keeps, so candidates whose counts differ from the plain run's are
@@ -64,21 +65,25 @@ bool same_counts(const sqd_counts& a, const sqd_counts& b) {
            a.num_critical_cuboids == b.num_critical_cuboids;
 }
 
-// Median time of `reps` proofs of [lo, hi) on a fresh core with settings s
-// (one untimed proof first, which also captures the graphs).
-Trial time_settings(int device, const sqd_program* prog, const sqd_settings& s, uint32_t lo, uint32_t hi, int reps) {
+// Median time of the proofs of [lo, hi) on a fresh core with settings s: two
+// untimed proofs first (the second captures the graphs), then at least `reps`
+// and as many as fit in about `budget` seconds (at most 200).
+Trial time_settings(int device, const sqd_program* prog, const sqd_settings& s, uint32_t lo, uint32_t hi, int reps,
+                    double budget) {
     Trial t;
     t.s = s;
     sqd_core* core = nullptr;
     t.status = sqd_core_create(device, prog, &s, &core);
     if(t.status != SQD_OK) return t;
     std::vector<double> times;
-    for(int r = 0; r <= reps && t.status == SQD_OK; ++r) {
+    int want = reps;
+    for(int r = 0; r < want + 2 && t.status == SQD_OK; ++r) {
         size_t nc = 0;
         const auto t0 = Clock::now();
         t.status = sqd_prove_range(core, lo, hi, &t.counts, nullptr, nullptr, 0, &nc, nullptr);
-        const auto t1 = Clock::now();
-        if(r > 0) times.push_back(seconds_since(t0, t1));
+        const double dt = seconds_since(t0, Clock::now());
+        if(r == 1 && dt > 0) want = std::max(reps, std::min(200, static_cast<int>(budget / dt)));
+        if(r >= 2) times.push_back(dt);
     }
     sqd_core_destroy(core);
     if(t.status == SQD_OK) {
@@ -149,12 +154,12 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         if(rc != SQD_OK) return rc;
 
         int trials = 0;
-        Trial plain = time_settings(device, prog, s0, lo, hi, o.repetitions);
+        Trial plain = time_settings(device, prog, s0, lo, hi, o.repetitions, o.target_seconds);
         ++trials;
         if(plain.status != SQD_OK) return plain.status;
         Trial cur = plain;
         auto consider = [&](sqd_settings s) {
-            Trial t = time_settings(device, prog, s, lo, hi, o.repetitions);
+            Trial t = time_settings(device, prog, s, lo, hi, o.repetitions, o.target_seconds);
             ++trials;
             if(t.status == SQD_OK && same_counts(t.counts, plain.counts) && t.seconds < cur.seconds) cur = t;
         };
@@ -166,14 +171,23 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
             s.fp32_prefilter = m;
             consider(s);
         }
-        // 2. dynamic cuboids per round
+        // 2. children per lane of the prefilter (when a level uses it)
+        if(cur.s.fp32_prefilter) {
+            const sqd_settings s_p = cur.s;
+            for(uint32_t g : {1u, 2u, 4u, 8u}) {
+                sqd_settings s = s_p;
+                s.prefilter_group = g;
+                consider(s);
+            }
+        }
+        // 3. dynamic cuboids per round
         const sqd_settings s_k = cur.s;
         for(uint32_t k : {1024u, 4096u, 16384u, 65536u}) {
             sqd_settings s = s_k;
             s.dequeue_buffer_size = k;
             consider(s);
         }
-        // 3. blocks per level launch (0: the kernel's resident capacity)
+        // 4. blocks per level launch (0: the kernel's resident capacity)
         const sqd_settings s_g = cur.s;
         for(uint32_t g : {cus, 2 * cus, 4 * cus}) {
             if(g == 0) continue;

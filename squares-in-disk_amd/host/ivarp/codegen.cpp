@@ -8,7 +8,9 @@
 #include <algorithm>
 #include <sstream>
 #include <stdexcept>
+#include <functional>
 #include <map>
+#include <set>
 #include <utility>
 #include <vector>
 #include <cstdio>
@@ -47,11 +49,23 @@ struct Emitter {
         const_index[node] = i;
         return i;
     }
+    // nodes whose values are read from the hoisted-term struct `h` instead of
+    // being computed (set while emitting a level's hoisted row check)
+    std::set<int> subst;
+
     void collect(int id, std::vector<int>& order, std::vector<char>& seen) {
         if(seen[static_cast<std::size_t>(id)]) return;
         seen[static_cast<std::size_t>(id)] = 1;
-        for(int k : p.nodes[static_cast<std::size_t>(id)].kids) collect(k, order, seen);
+        if(!subst.count(id))
+            for(int k : p.nodes[static_cast<std::size_t>(id)].kids) collect(k, order, seen);
         order.push_back(id);
+    }
+    bool is_pred(int id) const {
+        switch(p.nodes[static_cast<std::size_t>(id)].op) {
+            case FOp::BoolConst: case FOp::Lt: case FOp::Le: case FOp::Gt: case FOp::Ge: case FOp::Eq: case FOp::Neq:
+            case FOp::Not: case FOp::Or: case FOp::And: return true;
+            default: return false;
+        }
     }
     std::string v(int id) { return "v" + std::to_string(id); }
     std::string b(int id) { return "p" + std::to_string(id); }
@@ -65,17 +79,20 @@ struct Emitter {
     // the root to it.  Results are those of the full evaluation, since all nodes
     // are pure.
     using Ctx = std::vector<std::pair<int, int>>;
-    std::string eval(int root, const std::string& indent) {
+    std::string eval(int root, const std::string& indent) { return eval(std::vector<int>{root}, indent); }
+    // The same for several roots, all at the top scope.
+    std::string eval(const std::vector<int>& roots, const std::string& indent) {
         std::vector<int> order;
         std::vector<char> seen(p.nodes.size(), 0);
-        collect(root, order, seen);
+        for(int r : roots) collect(r, order, seen);
         std::map<int, Ctx> ctx;
-        ctx[root] = {};
+        for(int r : roots) ctx[r] = {};
         for(auto it = order.rbegin(); it != order.rend(); ++it) {
             const int id = *it;
             const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
             const Ctx& c = ctx.at(id);
             const bool lazy = n.op == FOp::Ite || n.op == FOp::Or || n.op == FOp::And;
+            if(subst.count(id)) continue;
             for(std::size_t i = 0; i < n.kids.size(); ++i) {
                 Ctx ck = c;
                 if(lazy && i > 0) ck.push_back({id, static_cast<int>(i)});
@@ -98,6 +115,11 @@ struct Emitter {
             if(ctx.at(id) != scope) continue;
             const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
             auto sub = [&](int i) { Ctx c = scope; c.push_back({id, i}); return c; };
+            if(subst.count(id)) {
+                if(is_pred(id)) o << indent << "const IB " << b(id) << " = h." << b(id) << ";\n";
+                else o << indent << "const IV " << v(id) << " = h." << v(id) << ";\n";
+                continue;
+            }
             if(n.op == FOp::Ite) {
                 const std::string c = b(n.kids[0]), t = "t" + std::to_string(id), e = "e" + std::to_string(id);
                 o << indent << "IV " << t << "{}, " << e << "{};\n";
@@ -281,10 +303,99 @@ std::string generate(const flat::Program& p, const std::string& key) {
     for(int c : p.rows[static_cast<std::size_t>(p.nargs - 1)]) body << "        if(!c" << c << "(lb, ub, C).poss) return false;\n";
     body << "        (void)lb; (void)ub; (void)C;\n        return true;\n    }\n";
 
+    // Hoisted row checks for the FP32 prefilter.  At static level L the rows
+    // are checked on a box whose variables other than the split variable A are
+    // the parent's for every child, so each term free of A has the same value
+    // for all children of a parent: hoist<L> computes the terms the other terms
+    // read (the frontier) once per parent into Hoist<L>, and
+    // rows_violated_hoisted<L> evaluates the rest per child, with the same
+    // results as rows_violated<L> on that box.
+    std::ostringstream hoist_types, hoist_fns;
+    hoist_fns << "    template<int L, class Num> __device__ static void hoist(const Num (&lb)[N], const Num (&ub)[N], "
+                 "const Num* C, Hoist<L, Num>& h) {\n        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C; (void)h;\n";
+    std::ostringstream rows_h;
+    rows_h << "    template<int L, class Num> __device__ static bool rows_violated_hoisted(const Num (&lb)[N], "
+              "const Num (&ub)[N], const Num* C, const Hoist<L, Num>& h) {\n        using IV = IVT<Num>;\n"
+              "        (void)lb; (void)ub; (void)C; (void)h;\n";
+    std::string hoist_sel = "NoHoist";
+    for(std::size_t L = sta.size(); L-- > 0;) {
+        const int A = p.splits[static_cast<std::size_t>(sta[L])].arg;
+        int from = A;
+        int to = L + 1 < sta.size() ? p.splits[static_cast<std::size_t>(sta[L + 1])].arg : p.nargs;
+        std::vector<int> level;
+        for(int r = from; r < to; ++r)
+            for(int c : p.rows[static_cast<std::size_t>(r)]) level.push_back(c);
+        // dependence on A
+        std::vector<int> dep(p.nodes.size(), -1);
+        std::function<bool(int)> depends = [&](int id) -> bool {
+            int& d = dep[static_cast<std::size_t>(id)];
+            if(d >= 0) return d != 0;
+            const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
+            bool r = n.op == FOp::Arg && n.arg == A;
+            for(int k : n.kids) r = depends(k) || r;
+            d = r ? 1 : 0;
+            return r;
+        };
+        // frontier: A-free terms (not constants or variables) read by terms that depend on A, or whole A-free constraints
+        std::set<int> frontier;
+        std::set<int> visited;
+        std::function<void(int)> walk = [&](int id) {
+            if(!visited.insert(id).second) return;
+            const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
+            for(int k : n.kids) {
+                const flat::FNode& kn = p.nodes[static_cast<std::size_t>(k)];
+                if(!depends(k)) {
+                    if(kn.op != FOp::Const && kn.op != FOp::Arg && kn.op != FOp::BoolConst) frontier.insert(k);
+                } else {
+                    walk(k);
+                }
+            }
+        };
+        for(int c : level) {
+            if(depends(c)) walk(c);
+            else if(p.nodes[static_cast<std::size_t>(c)].op != FOp::BoolConst) frontier.insert(c);
+        }
+        const std::string hname = "HoistL" + std::to_string(L);
+        hoist_types << "template<class Num> struct " << hname << " {";
+        for(int f : frontier) {
+            if(e.is_pred(f)) hoist_types << " IB " << e.b(f) << ";";
+            else hoist_types << " IVT<Num> " << e.v(f) << ";";
+        }
+        hoist_types << " };\n";
+        hoist_sel = "std::conditional_t<L == " + std::to_string(L) + ", " + hname + "<Num>, " + hoist_sel + ">";
+        // hoist<L>: the frontier terms
+        hoist_fns << "        if constexpr(L == " << L << ") {\n";
+        if(!frontier.empty()) {
+            std::vector<int> roots(frontier.begin(), frontier.end());
+            hoist_fns << e.eval(roots, "            ");
+            for(int f : frontier)
+                hoist_fns << "            h." << (e.is_pred(f) ? e.b(f) : e.v(f)) << " = " << (e.is_pred(f) ? e.b(f) : e.v(f)) << ";\n";
+        }
+        hoist_fns << "        }\n";
+        // rows_violated_hoisted<L>
+        rows_h << "        if constexpr(L == " << L << ") {\n";
+        e.subst = frontier;
+        for(int c : level) {
+            if(frontier.count(c)) {
+                rows_h << "            if(!h." << e.b(c) << ".poss) return true;\n";
+            } else {
+                rows_h << "            {\n" << e.eval(c, "                ");
+                rows_h << "                if(!" << e.b(c) << ".poss) return true;\n            }\n";
+            }
+        }
+        e.subst.clear();
+        rows_h << "        }\n";
+    }
+    hoist_fns << "    }\n";
+    rows_h << "        return false;\n    }\n";
+    body << "    template<int L, class Num> using Hoist = " << hoist_sel << ";\n";
+    body << hoist_fns.str() << rows_h.str();
+
     std::ostringstream o;
     o << "// Generated from the lowered program '" << key << "' (" << p.name << "); do not edit.\n";
     o << "#include \"kernel_set.hpp\"\n\n";
     o << "namespace " << ns << " {\nusing namespace sqd::dev;\n\n";
+    o << hoist_types.str() << "\n";
     o << "struct P {\n";
     o << "    static constexpr int N = " << p.nargs << ";\n";
     o << "    static constexpr int NB = " << p.bounds.size() << ";\n";

@@ -37,7 +37,7 @@ class Settings(ctypes.Structure):
     _fields_ = [("generation_count", ctypes.c_uint32), ("dequeue_buffer_size", ctypes.c_uint32),
                 ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
                 ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
-                ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("prefilter_group", ctypes.c_uint32)]
 
 
 FP32_ALL_LEVELS = 0xFFFFFFFF
