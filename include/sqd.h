@@ -279,7 +279,8 @@ int sqd_core_kernel_stats(const sqd_core* core, sqd_kernel_stat* out, size_t cap
 /* ---------------------------------------------------------------- tests */
 
 /* Elementwise directed rounding on the device (op: 0 add, 1 sub, 2 mul, 3 div,
- * 4 sqrt(a), 5 acos(a)): lo = RD(a op b), hi = RU(a op b).  Ops 8..13 are the
+ * 4 sqrt(a), 5 acos(a), 6 acos(a) by the fixed-point series alone, the fallback
+ * of 5): lo = RD(a op b), hi = RU(a op b).  Ops 8..13 are the
  * same in binary32 (add, sub, mul, div, sqrt, acos; operands must be floats,
  * results are floats widened exactly), op 14 the outward conversion of a to
  * binary32.  Arithmetic parity checks (acos: ivarp/src/interval_acos.cpp:29-66

@@ -1041,7 +1041,7 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
                 case 2: lo[i] = rd_mul(a[i], b[i]); hi[i] = ru_mul(a[i], b[i]); break;
                 case 3: lo[i] = rd_div(a[i], b[i]); hi[i] = ru_div(a[i], b[i]); break;
                 case 4: lo[i] = rd_sqrt(a[i]); hi[i] = ru_sqrt(a[i]); break;
-                case 5: lo[i] = acos_dir(a[i], false); hi[i] = acos_dir(a[i], true); break;
+                case 5: case 6: lo[i] = acos_dir(a[i], false); hi[i] = acos_dir(a[i], true); break;
                 case 8: case 9: case 10: case 11: case 12: {
                     const float x = static_cast<float>(a[i]), y = static_cast<float>(b[i]);
                     if(static_cast<double>(x) != a[i] || static_cast<double>(y) != b[i])

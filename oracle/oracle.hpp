@@ -70,7 +70,8 @@ int oracle_split(void* handle, std::uint64_t n, const double* lb, const double* 
                  double* out_lb, double* out_ub, std::uint64_t cap, std::uint64_t* n_out, OracleCounts* counts);
 
 // Elementwise directed-rounding primitives for arithmetic parity tests.
-// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 acos(a).  out_lo = RD(a op b), out_hi = RU(a op b).
+// op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt(a), 5 acos(a), 6 acos(a) (the same; the
+// device evaluates it by another method).  out_lo = RD(a op b), out_hi = RU(a op b).
 // Ops 8..13: the same in binary32 (operands must be floats; results widened
 // exactly), 14: outward conversion of a to binary32.
 int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b, double* out_lo, double* out_hi);

@@ -1,6 +1,11 @@
 // Directed-rounding arc cosine for binary64, in exact 128-bit fixed-point
 // integer arithmetic (independent of the floating-point rounding mode).
 //
+// Fast path (acos_directed): RD(acos(x)) is the largest double y with
+// cos(y) >= x; starting from an approximation, neighbouring doubles are tested
+// with a fixed-point cos (see cos_ge).  Fallback (acos_series), below: acos(x)
+// itself in fixed point.
+//
 // The reference rounds acos correctly in the required direction with MPFR
 // (ivarp/src/interval_acos.cpp:29-66): acos([l, u]) = [RD(acos(u)), RU(acos(l))].
 // Here acos(x) is evaluated as an integer Y = acos(x) * 2^124 with a proven error
@@ -134,15 +139,131 @@ __host__ __device__ inline double fx_up(u128 v, int z) {
     return ix ? from_bits_h(to_bits_h(d) + 1) : d;
 }
 
-// x in [-1, 1] (finite); returns RD(acos(x)) and RU(acos(x)); *hard is set when
-// the direction could not be decided (results widened).
+// ---------------------------------------------------------------- cos test
+// cos is strictly decreasing on [0, pi], so RD(acos(x)) is the largest double
+// y <= pi with cos(y) >= x.  cos(y) of a double y in [0, pi] is evaluated here
+// in the same fixed point: r = |y - pi/2| (exact but for the rounding of pi/2,
+// < 1 unit), cos(y) = -+sin(r), and sin(r) = sin(a) for a = r <= pi/4 or
+// cos(pi/2 - r) otherwise, so that every series argument is at most pi/4 and
+// z = argument^2 < 0.62.  The alternating series are summed from the inside,
+//   sin(a) = a (c0 - z (c1 - z (c2 - ...))),  c_k = 1/(2k+1)!
+//   cos(b) =    d0 - z (d1 - z (d2 - ...)),   d_k = 1/(2k)!,
+// where every partial sum lies in (0, c_k] (the terms decrease for z < 6), so
+// unsigned arithmetic suffices.  17 terms leave a tail below 2^-129; each step
+// truncates by less than 1 unit and the coefficients are floor(2^124 / n!)
+// exactly, and earlier errors shrink by z < 1, so the result is within
+// kCosErr units of the true value.
+constexpr u128 kCosErr = 64;
+constexpr int kCosTerms = 17;
+
+// floor(2^124 / n!) (nested floor divisions of an integer equal one floor division)
+__host__ __device__ constexpr u128 inv_factorial(int n) {
+    u128 v = kUnit << kF;
+    for(int i = 2; i <= n; ++i) v /= static_cast<u128>(i);
+    return v;
+}
+struct TrigCoeffs {
+    u128 s[kCosTerms];   // floor(2^124 / (2k+1)!)
+    u128 c[kCosTerms];   // floor(2^124 / (2k)!)
+};
+__host__ __device__ constexpr TrigCoeffs trig_coeffs() {
+    TrigCoeffs t{};
+    for(int k = 0; k < kCosTerms; ++k) {
+        t.s[k] = inv_factorial(2 * k + 1);
+        t.c[k] = inv_factorial(2 * k);
+    }
+    return t;
+}
+
+// sin(A / 2^124) * 2^124 for A <= pi/4 * 2^124, within kCosErr units
+__host__ __device__ inline u128 sin_fx(u128 a) {
+    constexpr TrigCoeffs T = trig_coeffs();
+    const u128 z = mulq(a, a);
+    u128 s = T.s[kCosTerms - 1];
+    for(int k = kCosTerms - 2; k >= 0; --k) s = T.s[k] - mulq(z, s);
+    return mulq(a, s);
+}
+// cos(B / 2^124) * 2^124 for B <= pi/4 * 2^124, within kCosErr units
+__host__ __device__ inline u128 cos_fx(u128 b) {
+    constexpr TrigCoeffs T = trig_coeffs();
+    const u128 z = mulq(b, b);
+    u128 s = T.c[kCosTerms - 1];
+    for(int k = kCosTerms - 2; k >= 0; --k) s = T.c[k] - mulq(z, s);
+    return s;
+}
+
+using i128 = __int128;
+
+// x as a signed fixed-point value, truncated toward zero (exact for |x| >= 2^-71)
+__host__ __device__ inline i128 to_fx_signed(double x) {
+    const std::uint64_t b = to_bits_h(x);
+    const int ex = static_cast<int>((b >> 52) & 0x7ff);
+    const std::uint64_t m = ex ? ((b & 0x000fffffffffffffull) | 0x0010000000000000ull) : (b & 0x000fffffffffffffull);
+    const int sh = (ex ? ex : 1) - 1075 + kF;
+    const u128 X = sh >= 0 ? (u128(m) << sh) : (sh > -64 ? u128(m >> (-sh)) : u128(0));
+    return (b >> 63) ? -static_cast<i128>(X) : static_cast<i128>(X);
+}
+
+// 1 if cos(y) >= x, 0 if cos(y) < x, -1 if undecided at this precision
+// (y a double in [0, pi] with y = 0 or y >= 2^-71, x in [-1, 1]).
+__host__ __device__ inline int cos_ge(double y, double x) {
+    const u128 Y = static_cast<u128>(to_fx_signed(y));
+    const u128 H = half_pi_fx();
+    const bool neg = Y >= H;                        // y >= pi/2 (up to the rounding of pi/2)
+    const u128 r = neg ? Y - H : H - Y;
+    const u128 Q = H >> 1;                          // pi/4
+    u128 s;
+    if(r <= Q) s = sin_fx(r);
+    else s = cos_fx(r <= H ? H - r : r - H);        // cos is even: |pi/2 - r|
+    const i128 c = neg ? -static_cast<i128>(s) : static_cast<i128>(s);
+    const i128 X = to_fx_signed(x);
+    const i128 e = static_cast<i128>(kCosErr + 2);  // series, pi/2 rounding, truncation of x
+    if(c - e > X) return 1;
+    if(c + e < X) return 0;
+    return -1;
+}
+
+// RD(acos(x)) for x in (-1, 1) by walking from an approximation y0 with cos_ge;
+// false when the walk does not settle within a few steps or a comparison is
+// undecided (the caller then uses the series below).
+__host__ __device__ inline bool acos_walk(double x, double y0, double* lo) {
+    constexpr std::uint64_t kPiDown = 0x400921fb54442d18ull;   // largest double below pi
+    const double pi_down = from_bits_h(kPiDown);
+    double y = y0;
+    if(!(y >= 0x1p-60)) y = 0x1p-60;                 // also catches NaN
+    if(y > pi_down) y = pi_down;
+    int t = cos_ge(y, x);
+    if(t < 0) return false;
+    if(t == 1) {                                      // y <= acos(x): step up while cos stays >= x
+        for(int i = 0; i < 8; ++i) {
+            if(y == pi_down) { *lo = y; return true; }
+            const double yn = from_bits_h(to_bits_h(y) + 1);
+            const int tn = cos_ge(yn, x);
+            if(tn < 0) return false;
+            if(tn == 0) { *lo = y; return true; }
+            y = yn;
+        }
+        return false;
+    }
+    for(int i = 0; i < 8; ++i) {                      // y > acos(x): step down until cos(y) >= x
+        y = from_bits_h(to_bits_h(y) - 1);
+        if(!(y >= 0x1p-60)) return false;
+        const int tn = cos_ge(y, x);
+        if(tn < 0) return false;
+        if(tn == 1) { *lo = y; return true; }
+    }
+    return false;
+}
+
+// x in [-1, 1] (finite); returns RD(acos(x)) and RU(acos(x)) from the series
+// alone; *hard is set when the direction could not be decided (results widened).
 //   |x| <= 1/2: Y = (pi/2 -+ asin|x|) * 2^124, absolute error < E.
 //   |x| >  1/2: u = (1 - |x|) / 2 (exact), scaled by 4^z so that the square root
 //       carries 122 significant bits; Y = 2 sqrt(u) series(u) * 2^(124 + z) is
 //       below the true value by less than E relative to its leading bit
 //       (relative error < 2^-114, whatever the magnitude of acos), then for x < 0
 //       acos(x) = pi - acos(|x|) at scale 2^124.
-__host__ __device__ inline void acos_directed(double x, double* lo, double* hi, bool* hard) {
+__host__ __device__ inline void acos_series(double x, double* lo, double* hi, bool* hard) {
     *hard = false;
     if(!(x >= -1.0 && x <= 1.0)) { *lo = *hi = __builtin_nan(""); return; }   // outside the domain, NaN
     if(x == 1.0) { *lo = 0.0; *hi = 0.0; return; }
@@ -182,6 +303,22 @@ __host__ __device__ inline void acos_directed(double x, double* lo, double* hi, 
     *lo = dl;
     *hi = uh;
     *hard = (dl != dh) || (ul != uh);
+}
+
+// RD(acos(x)) and RU(acos(x)) for x in [-1, 1]: the cos walk from the math
+// library's acos (any approximation works: the walk decides), else the series.
+// acos(x) is irrational for x != 1, so RU is the next double above RD.
+__host__ __device__ inline void acos_directed(double x, double* lo, double* hi, bool* hard) {
+    *hard = false;
+    if(x > -1.0 && x < 1.0) {
+        double l;
+        if(acos_walk(x, ::acos(x), &l)) {
+            *lo = l;
+            *hi = from_bits_h(to_bits_h(l) + 1);
+            return;
+        }
+    }
+    acos_series(x, lo, hi, hard);
 }
 
 } // namespace acosfx

@@ -21,6 +21,7 @@ __global__ void k_rounded_op(int op, unsigned long long n, const double* a, cons
             case 3: l = rd_div(x, y); h = ru_div(x, y); break;
             case 4: l = rd_sqrt(x); h = ru_sqrt(x); break;
             case 5: { bool hard; acosfx::acos_directed(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
+            case 6: { bool hard; acosfx::acos_series(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
             case 13: {   // binary32 acos: outward conversion of the binary64 directed results
                 const IVT<float> r = iv_acos<false, false>(IVT<float>{static_cast<float>(x), static_cast<float>(x), false});
                 l = r.lb; h = r.ub;
@@ -102,7 +103,7 @@ extern "C" int sqd_interval_op(int device, int op, size_t n, const double* a, co
 }
 
 extern "C" int sqd_rounded_op(int device, int op, size_t n, const double* a, const double* b, double* lo, double* hi) {
-    if(op < 0 || (op > 5 && op < 8) || op > 14 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
+    if(op < 0 || op == 7 || op > 14 || !a || !b || !lo || !hi) return SQD_ERR_ARGUMENT;
     if(n == 0) return SQD_OK;
     double *da = nullptr, *db = nullptr, *dl = nullptr, *dh = nullptr;
     const size_t bytes = n * sizeof(double);

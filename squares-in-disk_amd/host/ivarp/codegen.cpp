@@ -109,6 +109,33 @@ struct Emitter {
         emit_scope(order, ctx, {}, indent, o);
         return o.str();
     }
+    // Weighted size of the code emitted in `scope` (and the scopes nested in it):
+    // roughly the instructions it costs; square roots, divisions and acos weigh
+    // more than the rest.
+    int scope_cost(const std::vector<int>& order, const std::map<int, Ctx>& ctx, const Ctx& scope) const {
+        int cost = 0;
+        for(int id : order) {
+            const Ctx& c = ctx.at(id);
+            if(c.size() < scope.size() || !std::equal(scope.begin(), scope.end(), c.begin())) continue;
+            if(subst.count(id)) continue;
+            switch(p.nodes[static_cast<std::size_t>(id)].op) {
+                case FOp::Const: case FOp::Arg: case FOp::BoolConst: break;
+                case FOp::Sqrt: cost += 12; break;
+                case FOp::Div: cost += 16; break;
+                case FOp::Acos: cost += 200; break;
+                case FOp::Mul: case FOp::Square: cost += 3; break;
+                default: cost += 1; break;
+            }
+        }
+        return cost;
+    }
+    // Lazy nodes whose skipped operands cost less than this are evaluated in
+    // full instead: where the lanes of a wave disagree on the condition both
+    // branches run anyway, and the branch itself (exec-mask bookkeeping) costs
+    // more than a few interval operations.  All terms are pure, so the value is
+    // the same either way.
+    static constexpr int kEagerCost = 12;
+
     void emit_scope(const std::vector<int>& order, const std::map<int, Ctx>& ctx, const Ctx& scope,
                     const std::string& indent, std::ostringstream& o) {
         for(int id : order) {
@@ -118,6 +145,13 @@ struct Emitter {
             if(subst.count(id)) {
                 if(is_pred(id)) o << indent << "const IB " << b(id) << " = h." << b(id) << ";\n";
                 else o << indent << "const IV " << v(id) << " = h." << v(id) << ";\n";
+                continue;
+            }
+            if(n.op == FOp::Ite && scope_cost(order, ctx, sub(1)) + scope_cost(order, ctx, sub(2)) < kEagerCost) {
+                emit_scope(order, ctx, sub(1), indent, o);
+                emit_scope(order, ctx, sub(2), indent, o);
+                o << indent << "const IV " << v(id) << " = iv_ite(" << b(n.kids[0]) << ", " << v(n.kids[1]) << ", "
+                  << v(n.kids[2]) << ");\n";
                 continue;
             }
             if(n.op == FOp::Ite) {
@@ -134,6 +168,15 @@ struct Emitter {
             }
             if(n.op == FOp::Or || n.op == FOp::And) {
                 const bool is_or = n.op == FOp::Or;
+                int skipped = 0;
+                for(std::size_t i = 1; i < n.kids.size(); ++i) skipped += scope_cost(order, ctx, sub(static_cast<int>(i)));
+                if(skipped < kEagerCost) {
+                    for(std::size_t i = 1; i < n.kids.size(); ++i) emit_scope(order, ctx, sub(static_cast<int>(i)), indent, o);
+                    o << indent << "IB " << b(id) << " = " << b(n.kids[0]) << ";\n";
+                    for(std::size_t i = 1; i < n.kids.size(); ++i)
+                        o << indent << b(id) << " = " << (is_or ? "ib_or(" : "ib_and(") << b(id) << ", " << b(n.kids[i]) << ");\n";
+                    continue;
+                }
                 o << indent << "IB " << b(id) << " = " << b(n.kids[0]) << ";\n";
                 for(std::size_t i = 1; i < n.kids.size(); ++i) {
                     o << indent << "if(" << (is_or ? "!" : "") << b(id) << (is_or ? ".def" : ".poss") << ") {\n";

@@ -384,7 +384,7 @@ class Core:
 
 def rounded_op(op, a, b, device=0):
     """Device RD/RU of a op b (op: 'add', 'sub', 'mul', 'div', 'sqrt')."""
-    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5, "add32": 8, "sub32": 9, "mul32": 10,
+    code = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "acos": 5, "acos_series": 6, "add32": 8, "sub32": 9, "mul32": 10,
             "div32": 11, "sqrt32": 12, "acos32": 13, "to32": 14}[op]
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)

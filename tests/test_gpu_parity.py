@@ -71,6 +71,17 @@ def test_acos_bit_exact_with_mpfr():
     assert np.all(np.isnan(out[0])) and np.all(np.isnan(out[1]))
 
 
+@pytest.mark.parametrize("op", ["acos", "acos_series"])
+def test_acos_paths_bit_exact_with_mpfr(op):
+    # both device methods: the cos walk (with the series as its fallback) and the series alone
+    x = _acos_inputs(17, 1 << 16)
+    before = sqd.acos_fallbacks()
+    glo, ghi = sqd.rounded_op(op, x, x)
+    olo, ohi = pyoracle.rounded_op(op, x, x)
+    assert np.array_equal(_bits(glo), _bits(olo)) and np.array_equal(_bits(ghi), _bits(ohi))
+    assert sqd.acos_fallbacks() == before
+
+
 def test_acos_interval_domain_clamp():
     # interval_asin.hpp:13-27: with unbounded compile-time bounds the argument is clamped to
     # [-1, 1] and marked possibly undefined
