@@ -152,6 +152,28 @@ int sqd_prove_range(sqd_core* core, uint32_t first, uint32_t last, sqd_counts* c
  * to its on_critical callback (prover/proof_driver.hpp:499-509). */
 int sqd_core_criticals(sqd_core* core, double* crit_lb, double* crit_ub, size_t capacity, size_t* n);
 
+/* Progress of sqd_prove / sqd_prove_range: the reference's ProgressInfo and
+ * observer calls (prover/proof_driver.hpp:551-603, progress_printer.hpp:47-90):
+ * after `start_after_seconds`, then at most every `every_seconds`, between two
+ * rounds, and once more when the proof ends (done = 1, with its status).  The
+ * callback runs on the calling thread; head_lb/head_ub (N values each) hold the
+ * next cuboid of the frontier and are valid during the call only. */
+typedef struct sqd_progress {
+    uint64_t queue_size;          /* cuboids on the device frontier */
+    uint64_t cuboid_count;
+    uint64_t leaf_count;
+    uint64_t critical_count;
+    uint64_t rounds;              /* rounds completed */
+    int32_t done;                 /* 1 on the last call of a proof */
+    int32_t status;               /* SQD_OK, or the error that ended the proof */
+    const double* head_lb;        /* top of the frontier; NULL when it is empty */
+    const double* head_ub;
+} sqd_progress;
+typedef void (*sqd_progress_fn)(void* user, const sqd_progress* progress);
+/* fn = NULL turns progress reports off. */
+int sqd_core_set_progress(sqd_core* core, sqd_progress_fn fn, void* user, double start_after_seconds,
+                          double every_seconds);
+
 /* Evaluations of acos by this core's program whose rounding direction could not
  * be decided at 2^-113 precision, so that the interval was widened by one double
  * (still an enclosure); counted since the library was loaded.  Zero means every

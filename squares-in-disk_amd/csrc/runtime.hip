@@ -239,6 +239,10 @@ struct sqd_core {
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
     std::uint64_t device_bytes = 0;  // device memory allocated by sqd_core_create
+    sqd_progress_fn progress_fn = nullptr;   // progress reports of sqd_prove
+    void* progress_user = nullptr;
+    double progress_start = 0, progress_every = 0;
+    std::vector<double> head;        // 2N: the frontier's top for progress reports
     bool fp32 = false;               // some static level runs the FP32-prefiltered kernels
     // sqd_prove as HIP graphs: the initial queue with the first round, and one later round
     hipGraphExec_t g_first = nullptr, g_round = nullptr;
@@ -529,6 +533,34 @@ unsigned long long full_range(const sqd_core* c) { return c->dyn_initial.empty()
 void enqueue_initial_queue(sqd_core* c) {
     launch_initial_box(c, true, true);
     run_dynamic(c, dev::INIT, c->scratch.view(), c->scratch.count, 1, c->stack.view());
+}
+
+// Calls the core's progress callback with the last state brought to the host.
+void report_progress(sqd_core* c, unsigned long long rounds, bool done, int status) {
+    if(!c->progress_fn) return;
+    const DeviceState& h = *c->h_state;
+    sqd_progress pr{};
+    pr.queue_size = done && status != SQD_OK ? 0 : h.stack_count;
+    pr.cuboid_count = h.counters.cuboids;
+    pr.leaf_count = h.counters.leaves;
+    pr.critical_count = h.counters.criticals;
+    pr.rounds = rounds;
+    pr.done = done ? 1 : 0;
+    pr.status = status;
+    if(!done && h.stack_count > 0) {
+        c->head.assign(2 * static_cast<std::size_t>(c->N), 0.0);
+        const unsigned long long top = h.stack_count - 1;
+        for(int a = 0; a < c->N; ++a) {
+            hip_check(hipMemcpyAsync(&c->head[static_cast<std::size_t>(a)], c->stack.lb + static_cast<std::size_t>(a) * c->stack.cap + top,
+                                     sizeof(double), hipMemcpyDeviceToHost, c->stream), "progress head");
+            hip_check(hipMemcpyAsync(&c->head[static_cast<std::size_t>(c->N + a)], c->stack.ub + static_cast<std::size_t>(a) * c->stack.cap + top,
+                                     sizeof(double), hipMemcpyDeviceToHost, c->stream), "progress head");
+        }
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        pr.head_lb = c->head.data();
+        pr.head_ub = c->head.data() + c->N;
+    }
+    c->progress_fn(c->progress_user, &pr);
 }
 
 // Captures the launches and copies that `enqueue` makes on the core's stream
@@ -894,7 +926,7 @@ int sqd_prove(sqd_core* c, sqd_counts* counts, double* crit_lb, double* crit_ub,
 
 int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* counts, double* crit_lb,
                     double* crit_ub, size_t capacity, size_t* n_critical, double* seconds) {
-    return guarded([&] {
+    const int rc = guarded([&] {
         if(!c) throw std::invalid_argument("null core");
         const uint32_t all = c->dyn_initial.empty() ? 1u : static_cast<uint32_t>(c->dyn_initial[0]);
         if(first >= last || last > all)
@@ -913,22 +945,40 @@ int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* coun
             c->g_round = capture_graph(c, [&] { enqueue_round(c); enqueue_state_copy(c); });
         }
         const DeviceState* h = nullptr;
+        unsigned long long rounds = 0;
+        auto last_report = std::chrono::steady_clock::now();
+        bool reported = false;
+        auto progress = [&] {                      // between rounds, when due
+            if(!c->progress_fn) return;
+            const auto now = std::chrono::steady_clock::now();
+            if(std::chrono::duration<double>(now - t0).count() < c->progress_start) return;
+            if(reported && std::chrono::duration<double>(now - last_report).count() < c->progress_every) return;
+            reported = true;
+            last_report = now;
+            report_progress(c, rounds, false, SQD_OK);
+        };
         if(graphs) {
             hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
             h = &finish_state(c);
+            ++rounds;
             while(h->stack_count > 0) {
+                progress();
                 hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
                 h = &finish_state(c);
+                ++rounds;
             }
         } else {
             enqueue_initial_queue(c);
             do {
+                if(rounds > 0) progress();
                 enqueue_round(c);
                 h = &read_state(c);
+                ++rounds;
             } while(h->stack_count > 0);
         }
         ++c->proofs;
         c->dirty = false;
+        report_progress(c, rounds, true, SQD_OK);
         const sqd_counts r = counts_of(*h);
         const unsigned long long k = h->crit_count;
         auto t1 = std::chrono::steady_clock::now();
@@ -936,6 +986,23 @@ int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* coun
         if(counts) *counts = r;
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+    });
+    if(rc != SQD_OK && c && c->progress_fn) {     // the reference's observe_error
+        const std::string msg = t_error;
+        try { report_progress(c, 0, true, rc); } catch(...) {}
+        t_error = msg;
+    }
+    return rc;
+}
+
+int sqd_core_set_progress(sqd_core* c, sqd_progress_fn fn, void* user, double start_after_seconds,
+                          double every_seconds) {
+    return guarded([&] {
+        if(!c) throw std::invalid_argument("null core");
+        c->progress_fn = fn;
+        c->progress_user = user;
+        c->progress_start = start_after_seconds;
+        c->progress_every = every_seconds;
     });
 }
 

@@ -75,6 +75,16 @@ class JobResult(ctypes.Structure):
                 "latency_seconds": self.latency_seconds, "status": int(self.status)}
 
 
+class Progress(ctypes.Structure):
+    _fields_ = [("queue_size", ctypes.c_uint64), ("cuboid_count", ctypes.c_uint64), ("leaf_count", ctypes.c_uint64),
+                ("critical_count", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("done", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("head_lb", ctypes.POINTER(ctypes.c_double)),
+                ("head_ub", ctypes.POINTER(ctypes.c_double))]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Progress))
+
+
 class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("mode", ctypes.c_int32), ("level", ctypes.c_int32),
                 ("launches", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
@@ -136,6 +146,7 @@ def lib():
             "sqd_interval_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
             "sqd_test_acos_fallbacks": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_core_acos_fallbacks": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_core_set_progress": (ctypes.c_int, [_P, PROGRESS_FN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
             "sqd_core_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_width_stats": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(WidthStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_last_error": (ctypes.c_char_p, []),
@@ -159,6 +170,7 @@ def exported_symbols():
             "sqd_service_wait", "sqd_service_destroy",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
+            "sqd_core_set_progress",
             "sqd_rounded_op", "sqd_interval_op",
             "sqd_test_acos_fallbacks", "sqd_last_error"]
 
@@ -327,6 +339,25 @@ class Core:
         _check(lib().sqd_prove_range(self._h, first, last, ctypes.byref(c), None, None, 0, ctypes.byref(k),
                                      ctypes.byref(secs)))
         return {"counts": c.as_dict(), "n_critical": int(k.value), "seconds": secs.value}
+
+    def set_progress(self, fn, start_after=0.0, every=0.0):
+        """fn(dict) after each due round and at the end of sqd_prove (None: off)."""
+        if fn is None:
+            self._progress = None
+            _check(lib().sqd_core_set_progress(self._h, PROGRESS_FN(), None, 0.0, 0.0))
+            return
+        n = self.N
+
+        def cb(_user, p):
+            q = p.contents
+            head = None
+            if q.head_lb:
+                head = ([q.head_lb[a] for a in range(n)], [q.head_ub[a] for a in range(n)])
+            fn({"queue_size": int(q.queue_size), "cuboid_count": int(q.cuboid_count),
+                "leaf_count": int(q.leaf_count), "critical_count": int(q.critical_count), "rounds": int(q.rounds),
+                "done": bool(q.done), "status": int(q.status), "head": head})
+        self._progress = PROGRESS_FN(cb)              # kept alive while registered
+        _check(lib().sqd_core_set_progress(self._h, self._progress, None, start_after, every))
 
     def shape(self):
         """(initial subdivisions of the first dynamic variable, static levels)."""

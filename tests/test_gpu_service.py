@@ -121,3 +121,23 @@ def test_autotune_keeps_counts():
         assert r["counts"] == rep["counts"]
         full = core.prove()["counts"]
         assert (full["num_cuboids"], full["num_leaf_cuboids"]) == PUBLISHED["three"]
+
+
+def test_progress_reports():
+    # progress between rounds and once at the end (ProgressInfo of the reference)
+    p = sqd.Program.from_lemma("more1")                    # about 160 rounds
+    seen = []
+    with sqd.Core(p) as core:
+        core.set_progress(seen.append, start_after=0.0, every=0.0)
+        r = core.prove()
+        core.set_progress(None)
+        r2 = core.prove()
+    assert r2["counts"] == r["counts"]
+    assert len(seen) >= 3 and seen[-1]["done"] and seen[-1]["status"] == 0
+    assert all(not s["done"] for s in seen[:-1])
+    assert (seen[-1]["cuboid_count"], seen[-1]["leaf_count"]) == PUBLISHED["more1"] and seen[-1]["queue_size"] == 0
+    cub = [s["cuboid_count"] for s in seen]
+    assert cub == sorted(cub) and [s["rounds"] for s in seen] == sorted(s["rounds"] for s in seen)
+    for s in seen[:-1]:
+        lb, ub = s["head"]
+        assert s["queue_size"] > 0 and all(a <= b for a, b in zip(lb, ub))
