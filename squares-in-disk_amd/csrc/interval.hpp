@@ -72,12 +72,29 @@ __device__ inline IVT<X> iv_mul(IVT<X> a, IVT<X> b) {
     return IVT<X>{lo, hi, a.und || b.und};
 }
 
-// square(x) = x * x with run-time sign checks, then bounded below by 0.
+// square(x) = x * x with run-time sign checks, then bounded below by 0
+// (math_fn/fixed_pow.hpp:183-187).  The product's sign cases with both operands
+// equal reduce to: lb >= 0: [lb^2, ub^2]; ub <= 0: [ub^2, lb^2]; otherwise the
+// lower bound lb*ub < 0 is raised to +0 and the upper bound is the larger of
+// lb^2 and ub^2 (same comparison as the general product, so the same bits).
 template<class X>
 __device__ inline IVT<X> iv_square(IVT<X> a) {
-    IVT<X> r = iv_mul<R, R, R, R, false>(a, a);
-    if(r.lb < X(0)) r.lb = X(0);
-    return r;
+    X lo, hi;
+    if(a.lb >= X(0)) {
+        lo = rd_mul(a.lb, a.lb);
+        hi = ru_mul(a.ub, a.ub);
+    } else if(a.ub <= X(0)) {
+        lo = rd_mul(a.ub, a.ub);
+        hi = ru_mul(a.lb, a.lb);
+    } else {
+        const X lh = nan_to_zero(ru_mul(a.lb, a.lb)), uh = nan_to_zero(ru_mul(a.ub, a.ub));
+        lo = X(0);
+        hi = (lh <= uh) ? uh : lh;
+    }
+    lo = nan_to_zero(lo);
+    hi = nan_to_zero(hi);
+    if(lo < X(0)) lo = X(0);
+    return IVT<X>{lo, hi, a.und};
 }
 
 template<class X> __device__ __forceinline__ int sgn(X v) { return (v > X(0)) - (v < X(0)); }
