@@ -395,9 +395,80 @@ __device__ __forceinline__ void static_decide(const LevelArgs& args, unsigned lo
     }
 }
 
+// The binary64 decision of child i of the parent (plb, pub), whose terms free
+// of the level's variable were hoisted into h (in LDS): as static_decide, but
+// the rows are checked with the hoisted terms when propagation left every other
+// variable bit for bit as in the parent (the child's box is then, in every
+// variable those terms read, the box h was computed for), and in full otherwise.
+// Called by every lane of the wave.
+template<class P, int MODE, int L>
+__device__ __forceinline__ void static_decide_hoisted(const LevelArgs& args, bool valid, const double (&plb)[P::N],
+                                                      const double (&pub)[P::N], double pl, double pu, double wd, int i,
+                                                      unsigned origin, unsigned leaf,
+                                                      const typename P::template Hoist<L, double>& h,
+                                                      const double* s_c, PropScratch<P>& s_prop,
+                                                      unsigned long long& c_leaf, unsigned long long& c_crit) {
+    constexpr int N = P::N;
+    constexpr int ARG = P::st_arg(L);
+    constexpr int SUBS = P::st_subs(L);
+    constexpr bool LAST = (L == P::NSTAT - 1);
+    bool survive = false, join = false;
+    double lb[N], ub[N];
+#pragma unroll
+    for(int a = 0; a < N; ++a) { lb[a] = plb[a]; ub[a] = pub[a]; }
+    if(valid) {
+        lb[ARG] = split_point(pl, pu, wd, i, SUBS);
+        ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
+        if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+            ++c_leaf;
+        } else {
+            bool same = true;
+#pragma unroll
+            for(int a = 0; a < N; ++a)
+                if(a != ARG) same = same && bits(lb[a]) == bits(plb[a]) && bits(ub[a]) == bits(pub[a]);
+            const bool violated = same ? P::template rows_violated_hoisted<L>(lb, ub, s_c, h)
+                                       : P::template rows_violated<L>(lb, ub, s_c);
+            if(violated) {
+                ++c_leaf;
+            } else if(LAST) {
+                ++c_leaf;
+                if(MODE == FINAL) { ++c_crit; survive = true; }
+                else join = true;
+            } else {
+                survive = true;
+            }
+        }
+    }
+    if(LAST && MODE == NONFINAL) hull_join<N>(args.hull, join, origin, leaf, lb, ub);
+    if(!LAST || MODE == FINAL) {
+        const unsigned long long pos = wave_append(survive, args.out.count);
+        if(survive) {
+            if(pos < args.out.cap) {
+                store_box<P>(args.out, pos, lb, ub);
+                if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
+            } else {
+                atomicAdd(&args.counters->overflow, 1ull);
+            }
+        }
+    }
+}
+
+// Whether the binary64 static level L hoists its split-free terms per wave:
+// when a wave can take all children of a parent in several trips of 64, the
+// level has such terms, and it is not the first static level (whose parents,
+// the round's dynamic cuboids, are usually too few to give every wave one).
+template<class P, int L>
+constexpr bool wave_hoisting() {
+    return L >= 1 && P::st_subs(L) % 64 == 0 && P::st_subs(L) >= 128 &&
+           sizeof(typename P::template Hoist<L, double>) > 1;
+}
+
 // Level L of the static grid (modes NONFINAL and FINAL).
 //
 // F32 = false: every child w of the level's parents is decided in binary64.
+// Where wave_hoisting<P, L>() holds, each wave takes whole parents: it computes
+// the parent's split-free terms once into its LDS slot, then decides the
+// SUBS children in trips of 64 (static_decide_hoisted).
 // F32 = true: the children listed by k_static_prefilter (those binary32 left
 // undecided) are decided in binary64, exactly as above.
 template<class P, int MODE, int L, bool F32>
@@ -416,14 +487,58 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
     unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
     // every lane of a wave runs the same number of loop trips, so the wave-wide
     // ballots always see all lanes of the wave
-    const unsigned long long trips = (total + stride - 1) / stride;
-    for(unsigned long long trip = 0; trip < trips; ++trip) {
-        const unsigned long long t = first + trip * stride;
-        const bool valid = t < total;
-        unsigned long long w = t;
-        if(F32) w = valid ? args.undec[t] : 0ull;
-        else if(valid && w % SUBS == 0) c_cub += SUBS;
-        static_decide<P, MODE, L>(args, in_base, valid, w, s_c, s_prop, c_leaf, c_crit);
+    if constexpr(!F32 && wave_hoisting<P, L>()) {
+        constexpr int N = P::N;
+        constexpr int ARG = P::st_arg(L);
+        using H = typename P::template Hoist<L, double>;
+        __shared__ H s_h[kBlock / 64];
+        H& h = s_h[threadIdx.x / 64];
+        const int lane = static_cast<int>(__lane_id());
+        const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x / 64);
+        const unsigned long long wave = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+        const unsigned long long wtrips = (n_items + waves - 1) / waves;   // the same for every wave
+        for(unsigned long long trip = 0; trip < wtrips; ++trip) {
+            const unsigned long long item = wave + trip * waves;
+            const bool valid = item < n_items;                            // wave-uniform
+            double plb[N], pub[N];
+            double pl = 0, pu = 0, wd = 0;
+            unsigned p_origin = 0, p_leaf = 0;
+#pragma unroll
+            for(int a = 0; a < N; ++a) { plb[a] = 0.0; pub[a] = 0.0; }
+            if(valid) {
+                const unsigned long long src = in_base + item;
+                load_box<P>(args.in, src, plb, pub);
+                if(args.in.origin) { p_origin = args.in.origin[src]; p_leaf = args.in.leaf[src]; }
+                else { p_origin = static_cast<unsigned>(item); p_leaf = 0; }
+                pl = plb[ARG];
+                pu = pub[ARG];
+                wd = split_width(pl, pu, SUBS);
+                H hv;
+                P::template hoist<L>(plb, pub, s_c, hv);
+                if(lane == 0) { h = hv; c_cub += SUBS; }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll 1
+            for(int k = 0; k < SUBS / 64; ++k) {
+                const int i = lane + 64 * k;
+                static_decide_hoisted<P, MODE, L>(args, valid, plb, pub, pl, pu, wd, i, p_origin,
+                                                  p_leaf * SUBS + static_cast<unsigned>(i), h, s_c, s_prop, c_leaf,
+                                                  c_crit);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        const unsigned long long trips = (total + stride - 1) / stride;
+        for(unsigned long long trip = 0; trip < trips; ++trip) {
+            const unsigned long long t = first + trip * stride;
+            const bool valid = t < total;
+            unsigned long long w = t;
+            if(F32) w = valid ? args.undec[t] : 0ull;
+            else if(valid && w % SUBS == 0) c_cub += SUBS;
+            static_decide<P, MODE, L>(args, in_base, valid, w, s_c, s_prop, c_leaf, c_crit);
+        }
     }
     block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
 }
