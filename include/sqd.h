@@ -71,6 +71,10 @@ typedef struct sqd_settings {
     uint32_t prefilter_group;      /* FP32 prefilter: most children of one parent a lane
                                       evaluates together, sharing the terms free of the
                                       split variable (1, 2, 4, 8; 0 = chosen per launch) */
+    uint32_t jobs;                 /* jobs per batch (sqd_prove_batch), at most 64; the core
+                                      holds that many copies of its device memory and runs
+                                      every launch with one grid row per job; 0 or 1: one */
+    uint32_t reserved;
 } sqd_settings;
 
 #define SQD_FP32_ALL_LEVELS 0xffffffffu
@@ -145,6 +149,29 @@ int sqd_prove(sqd_core* core, sqd_counts* counts, double* crit_lb, double* crit_
  * of the proof service.  No counterpart in the reference. */
 int sqd_prove_range(sqd_core* core, uint32_t first, uint32_t last, sqd_counts* counts, double* crit_lb,
                     double* crit_ub, size_t capacity, size_t* n_critical, double* seconds);
+
+/* Several proof jobs of one program in the same launches (no counterpart in the
+ * reference, which proves one statement at a time): job j proves the initial
+ * range [first[j], last[j]) exactly as sqd_prove_range would, with its own
+ * frontier, level buffers, counters and criticals, in the core's job memory j.
+ * Every kernel of a round runs one grid row per job, so the per-launch cost and
+ * the small kernels of a round (initial queue, round control, count fold) are
+ * shared by the batch, and one job's idle rows leave the machine to the others.
+ * n_jobs <= sqd_settings.jobs of the core; the call ends when every job has.
+ * A job whose buffers overflow fails the whole call (SQD_ERR_CAPACITY).
+ * sqd_prove and sqd_prove_range on such a core run a batch of one. */
+typedef struct sqd_job_counts {
+    sqd_counts counts;
+    uint64_t n_critical;      /* criticals the job found (sqd_core_job_criticals) */
+    uint64_t fp32_checked;    /* static-level children the FP32 prefilter evaluated */
+    uint64_t fp32_decided;    /* ... and pruned */
+} sqd_job_counts;
+int sqd_prove_batch(sqd_core* core, size_t n_jobs, const uint32_t* first, const uint32_t* last,
+                    sqd_job_counts* results, double* seconds);
+/* The critical cuboids of job `job` of the last sqd_prove_batch (as
+ * sqd_core_criticals: up to capacity boxes, SoA stride capacity; *n the total). */
+int sqd_core_job_criticals(sqd_core* core, size_t job, double* crit_lb, double* crit_ub, size_t capacity,
+                           size_t* n);
 
 /* The critical cuboids found by the last sqd_prove on this core (they stay in
  * device memory until the next call): up to `capacity` boxes (SoA, stride
@@ -241,8 +268,10 @@ int sqd_autotune(int device, const sqd_program* program, const sqd_settings* bas
 
 /* A queue of proof jobs (a program and a range [first, last) of its first
  * variable's initial subdivisions, as in sqd_prove_range) run by worker threads,
- * each on a pooled core of the job's program with its own stream, so that the
- * kernels of up to `workers` jobs overlap on the device.  Cores are created on
+ * each on a pooled core of the job's program with its own stream.  A worker
+ * takes the oldest queued job together with up to batch_jobs - 1 later queued
+ * jobs of the same program and proves them as one batch (sqd_prove_batch); the
+ * batches of up to `workers` workers overlap on the device.  Cores are created on
  * demand within `memory_budget` bytes of device memory; idle cores of other
  * programs are destroyed to make room.  Jobs may be submitted from any thread
  * while others run.  Programs must outlive the service.  No counterpart in the
@@ -250,13 +279,16 @@ int sqd_autotune(int device, const sqd_program* program, const sqd_settings* bas
 typedef struct sqd_service sqd_service;
 typedef struct sqd_service_options {
     int32_t device;
-    uint32_t workers;             /* jobs in flight (default 8) */
-    uint64_t memory_budget;       /* device bytes for the pooled cores (default 64 GiB) */
+    uint32_t workers;             /* batches in flight (default 8) */
+    uint64_t memory_budget;       /* device bytes for the pooled cores (default 128 GiB) */
     uint32_t fp32_prefilter;      /* prefilter level mask of every core (default SQD_FP32_ALL_LEVELS) */
     uint32_t autotune;            /* nonzero: sqd_autotune each program on its first job */
     uint64_t level_bytes;         /* per core (default 256 MiB) */
     uint64_t frontier_capacity;   /* per core (default 1 << 22 cuboids) */
     uint64_t core_bytes_estimate; /* size assumed for a program's first core (default 2 GiB) */
+    uint32_t batch_jobs;          /* queued jobs of one program a worker proves together
+                                     (sqd_prove_batch; each core holds that many; default 8) */
+    uint32_t reserved;
 } sqd_service_options;
 typedef struct sqd_job_result {
     sqd_counts counts;

@@ -93,7 +93,50 @@ struct LevelArgs {
     const unsigned long long* init_range;
     // FP32 prefilter: most children of one parent per lane (0: chosen from the launch size)
     int children_per_lane;
+    // batched launches: the grid's row blockIdx.y works on job blockIdx.y, whose
+    // memory is job 0's shifted by blockIdx.y * job_stride bytes (see job_args);
+    // `jobs` is the number of rows the host launches
+    unsigned long long job_stride;
+    unsigned jobs;
 };
+
+// ---------------------------------------------------------------- batched jobs
+// A batch of jobs keeps every job's device memory (lists, control block, count
+// slots, hull records, constants) in one allocation of equal parts, so job j's
+// pointers are job 0's plus j times the part's size.  A kernel launched over a
+// grid of several rows shifts its pointer arguments by its row's offset; null
+// pointers (an absent list column, a missing input base) stay null.
+template<class T>
+__device__ __forceinline__ T* job_ptr(T* p, unsigned long long off) {
+    return p ? reinterpret_cast<T*>(reinterpret_cast<unsigned long long>(p) + off) : p;
+}
+__device__ __forceinline__ unsigned long long job_offset(unsigned long long stride) {
+    return static_cast<unsigned long long>(blockIdx.y) * stride;
+}
+__device__ __forceinline__ BoxList job_list(BoxList b, unsigned long long off) {
+    b.lb = job_ptr(b.lb, off);
+    b.ub = job_ptr(b.ub, off);
+    b.depth = job_ptr(b.depth, off);
+    b.origin = job_ptr(b.origin, off);
+    b.leaf = job_ptr(b.leaf, off);
+    b.count = job_ptr(b.count, off);
+    return b;
+}
+__device__ __forceinline__ LevelArgs job_args(LevelArgs a) {
+    const unsigned long long off = job_offset(a.job_stride);
+    a.in = job_list(a.in, off);
+    a.out = job_list(a.out, off);
+    a.in_base = job_ptr(a.in_base, off);
+    a.in_count = job_ptr(a.in_count, off);
+    a.hull = job_ptr(a.hull, off);
+    a.counters = job_ptr(a.counters, off);
+    a.block_counts = job_ptr(a.block_counts, off);
+    a.consts = job_ptr(a.consts, off);
+    a.undec = job_ptr(a.undec, off);
+    a.undec_count = job_ptr(a.undec_count, off);
+    a.init_range = job_ptr(a.init_range, off);
+    return a;
+}
 
 // ---------------------------------------------------------------- hull keys
 __device__ __forceinline__ unsigned long long order_key(double x) {
@@ -472,7 +515,8 @@ constexpr bool wave_hoisting() {
 // F32 = true: the children listed by k_static_prefilter (those binary32 left
 // undecided) are decided in binary64, exactly as above.
 template<class P, int MODE, int L, bool F32>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs args) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs job0) {
+    const LevelArgs args = job_args(job0);
     constexpr int SUBS = P::st_subs(L);
     const unsigned long long n_items = *args.in_count;
     const unsigned long long total = F32 ? *args.undec_count : n_items * static_cast<unsigned long long>(SUBS);
@@ -559,7 +603,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
 // The binary32 pass runs in its own kernel so that its register budget (and
 // with it the occupancy) is not that of the binary64 code.
 template<class P, int MODE, int L>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(LevelArgs args) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(LevelArgs job0) {
+    const LevelArgs args = job_args(job0);
     constexpr int N = P::N;
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
@@ -651,7 +696,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
 // initial queue).  The last level keeps children that possibly satisfy every
 // constraint and appends them to the output (the cuboid stack).
 template<class P, int MODE, int J>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelArgs args) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelArgs job0) {
+    const LevelArgs args = job_args(job0);
     constexpr int N = P::N;
     constexpr int ARG = P::dyn_arg(J);
     constexpr int SUBS = MODE == INIT ? P::dyn_initial(J) : P::dyn_subs(J);
@@ -727,13 +773,24 @@ __device__ __forceinline__ void clear_hull_record(unsigned long long* rec, int n
 // Turn the hull records of one round into boxes.  Hulls at depth >= generations
 // go to `fin` (final pass), the others to `spl` with depth + 1 (dynamic split).
 // Without static variables the popped cuboid itself is the only leaf.  The
-// round's items are [*base, *base + *count) of `items`.
+// round's items are [*base, *base + *count) of `items`.  Row blockIdx.y of the
+// grid works on job blockIdx.y.
 template<class P>
 __global__ __launch_bounds__(kBlock) void k_hull_finalize(BoxList items, const unsigned long long* base_ptr,
                                                           const unsigned long long* count_ptr, unsigned long long* hull,
                                                           BoxList fin, BoxList spl, DeviceCounters* counters,
-                                                          unsigned long long* block_counts, unsigned generations) {
+                                                          unsigned long long* block_counts, unsigned generations,
+                                                          unsigned long long job_stride) {
     constexpr int N = P::N;
+    const unsigned long long off = job_offset(job_stride);
+    items = job_list(items, off);
+    fin = job_list(fin, off);
+    spl = job_list(spl, off);
+    base_ptr = job_ptr(base_ptr, off);
+    count_ptr = job_ptr(count_ptr, off);
+    hull = job_ptr(hull, off);
+    counters = job_ptr(counters, off);
+    block_counts = job_ptr(block_counts, off);
     const unsigned long long n = *count_ptr, base = *base_ptr;
     const unsigned long long d = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool have = false;
@@ -803,9 +860,15 @@ __global__ __launch_bounds__(kBlock) void k_hull_decode(BoxList items, unsigned 
     has[d] = ok ? 1 : 0;
 }
 
-// Final pass without static variables: every cuboid is one critical leaf.
+// Final pass without static variables: every cuboid is one critical leaf.  Row
+// blockIdx.y of the grid works on job blockIdx.y.
 template<class P>
-__global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList out, unsigned long long* block_counts) {
+__global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList out, unsigned long long* block_counts,
+                                                           unsigned long long job_stride) {
+    const unsigned long long off = job_offset(job_stride);
+    fin = job_list(fin, off);
+    out = job_list(out, off);
+    block_counts = job_ptr(block_counts, off);
     const unsigned long long n = *fin.count;
     const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool v = i < n;
@@ -819,10 +882,19 @@ __global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList 
 // Initial runtime box: outward compile-time box, then 50*N rounds of all bounds.
 // First `zero_words` words at `zero` are cleared (a fresh control block).  The
 // box goes to `out` (N lb, N ub) and, when list.lb is set, to entry 0 of `list`
-// (depth 0) with the list's count set to 1.
+// (depth 0) with the list's count set to 1.  Row blockIdx.y of the grid works on
+// job blockIdx.y (every pointer shifted by its offset, as job_args does).
 template<class P>
 __global__ void k_initial_box(const double* box_lb, const double* box_ub, const double* consts, double* out,
-                              BoxList list, unsigned long long* zero, unsigned zero_words) {
+                              BoxList list, unsigned long long* zero, unsigned zero_words,
+                              unsigned long long job_stride) {
+    const unsigned long long off = job_offset(job_stride);
+    box_lb = job_ptr(box_lb, off);
+    box_ub = job_ptr(box_ub, off);
+    consts = job_ptr(consts, off);
+    out = job_ptr(out, off);
+    list = job_list(list, off);
+    zero = job_ptr(zero, off);
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
     for(unsigned i = threadIdx.x; zero && i < zero_words; i += blockDim.x) zero[i] = 0ull;
     __threadfence_block();

@@ -25,14 +25,18 @@ struct KernelSet {
     hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
                                 unsigned cus, unsigned grid, hipStream_t s);
     // Hulls of the round's items [*base, *base + *count) (at most max_n of them).
+    // These three launch `jobs` grid rows, row j on job j's memory (job 0's
+    // pointers plus j * job_stride bytes); the level launches take the same from
+    // LevelArgs.
     hipError_t (*hull_finalize)(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
                                 unsigned long long max_n, unsigned long long* hull, dev::BoxList fin,
                                 dev::BoxList spl, dev::DeviceCounters* c, unsigned long long* block_counts,
-                                unsigned gens, hipStream_t s);
+                                unsigned gens, unsigned jobs, unsigned long long job_stride, hipStream_t s);
     hipError_t (*final_nostatic)(dev::BoxList fin, dev::BoxList out, unsigned long long* block_counts, unsigned grid,
-                                 hipStream_t s);
+                                 unsigned jobs, unsigned long long job_stride, hipStream_t s);
     hipError_t (*initial_box)(const double* lb, const double* ub, const double* consts, double* out,
-                              dev::BoxList list, unsigned long long* zero, unsigned zero_words, hipStream_t s);
+                              dev::BoxList list, unsigned long long* zero, unsigned zero_words, unsigned jobs,
+                              unsigned long long job_stride, hipStream_t s);
     hipError_t (*hull_decode)(dev::BoxList items, unsigned long long n, unsigned long long* hull,
                               dev::BoxList out, unsigned char* has, hipStream_t s);
     // Undecided acos roundings of this program so far (device-wide counter).
@@ -57,6 +61,8 @@ unsigned blocks_per_cu(K kernel) {
     if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, dev::kBlock, 0) != hipSuccess || n < 1) n = 1;
     return static_cast<unsigned>(n);
 }
+// Rows of a level launch: one per job of the batch (job_args).
+inline unsigned jobs_of(const dev::LevelArgs& a) { return a.jobs ? a.jobs : 1u; }
 inline unsigned level_grid(unsigned resident, unsigned long long max_children, unsigned grid) {
     if(grid) return grid;
     const unsigned long long need = (max_children + dev::kBlock - 1) / dev::kBlock;
@@ -70,22 +76,22 @@ hipError_t launch_one_static(const dev::LevelArgs& a, unsigned long long max_chi
                              hipStream_t s) {
     if(F32) {
         static const unsigned pre_cu = blocks_per_cu(dev::k_static_prefilter<P, MODE, L>);
-        hipLaunchKernelGGL((dev::k_static_prefilter<P, MODE, L>), dim3(level_grid(pre_cu * cus, max_children, grid)),
+        hipLaunchKernelGGL((dev::k_static_prefilter<P, MODE, L>), dim3(level_grid(pre_cu * cus, max_children, grid), jobs_of(a)),
                            dim3(dev::kBlock), 0, s, a);
         const hipError_t e = hipGetLastError();
         if(e != hipSuccess) return e;
     }
     static const unsigned per_cu = blocks_per_cu(dev::k_static_level<P, MODE, L, F32>);
-    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L, F32>), dim3(level_grid(per_cu * cus, max_children, grid)),
-                       dim3(dev::kBlock), 0, s, a);
+    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L, F32>),
+                       dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
 template<class P, int MODE, int J>
 hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
                               hipStream_t s) {
     static const unsigned per_cu = blocks_per_cu(dev::k_dynamic_level<P, MODE, J>);
-    hipLaunchKernelGGL((dev::k_dynamic_level<P, MODE, J>), dim3(level_grid(per_cu * cus, max_children, grid)),
-                       dim3(dev::kBlock), 0, s, a);
+    hipLaunchKernelGGL((dev::k_dynamic_level<P, MODE, J>),
+                       dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
 template<class P, int MODE, bool F32, int... L>
@@ -127,24 +133,27 @@ hipError_t dynamic_level(int mode, int level, const dev::LevelArgs& a, unsigned 
 template<class P>
 hipError_t hull_finalize(dev::BoxList items, const unsigned long long* base, const unsigned long long* count,
                          unsigned long long max_n, unsigned long long* hull, dev::BoxList fin, dev::BoxList spl,
-                         dev::DeviceCounters* c, unsigned long long* block_counts, unsigned gens, hipStream_t s) {
+                         dev::DeviceCounters* c, unsigned long long* block_counts, unsigned gens, unsigned jobs,
+                         unsigned long long job_stride, hipStream_t s) {
     if(max_n == 0) return hipSuccess;
     unsigned grid = static_cast<unsigned>((max_n + dev::kBlock - 1) / dev::kBlock);
-    hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid), dim3(dev::kBlock), 0, s, items, base, count, hull, fin,
-                       spl, c, block_counts, gens);
+    hipLaunchKernelGGL((dev::k_hull_finalize<P>), dim3(grid, jobs ? jobs : 1u), dim3(dev::kBlock), 0, s, items, base,
+                       count, hull, fin, spl, c, block_counts, gens, job_stride);
     return hipGetLastError();
 }
 template<class P>
 hipError_t final_nostatic(dev::BoxList fin, dev::BoxList out, unsigned long long* block_counts, unsigned grid,
-                          hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_final_nostatic<P>), dim3(grid), dim3(dev::kBlock), 0, s, fin, out, block_counts);
+                          unsigned jobs, unsigned long long job_stride, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_final_nostatic<P>), dim3(grid, jobs ? jobs : 1u), dim3(dev::kBlock), 0, s, fin, out,
+                       block_counts, job_stride);
     return hipGetLastError();
 }
 template<class P>
 hipError_t initial_box(const double* lb, const double* ub, const double* consts, double* out, dev::BoxList list,
-                       unsigned long long* zero, unsigned zero_words, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1), dim3(dev::kBlock), 0, s, lb, ub, consts, out, list, zero,
-                       zero_words);
+                       unsigned long long* zero, unsigned zero_words, unsigned jobs, unsigned long long job_stride,
+                       hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_initial_box<P>), dim3(1, jobs ? jobs : 1u), dim3(dev::kBlock), 0, s, lb, ub, consts,
+                       out, list, zero, zero_words, job_stride);
     return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -78,19 +79,39 @@ static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
-__global__ void k_set_range(unsigned long long* p, unsigned long long lo, unsigned long long hi) {
-    if(threadIdx.x == 0 && blockIdx.x == 0) { p[0] = lo; p[1] = hi; }
+// Most jobs of one batch (sqd_prove_batch).
+constexpr unsigned kMaxJobs = 64;
+// The initial-queue range of each job of a batch: [lo[j], hi[j]).
+struct JobRanges {
+    unsigned lo[kMaxJobs];
+    unsigned hi[kMaxJobs];
+};
+// Kernels below with a `stride` argument run one grid row per job of a batch:
+// row blockIdx.y works on job blockIdx.y's memory (job 0's pointers shifted by
+// blockIdx.y * stride bytes).
+__global__ void k_set_ranges(unsigned long long* p, unsigned long long stride, JobRanges r) {
+    p = dev::job_ptr(p, dev::job_offset(stride));
+    if(threadIdx.x == 0 && blockIdx.x == 0) { p[0] = r.lo[blockIdx.y]; p[1] = r.hi[blockIdx.y]; }
 }
 // Start of a proof (or of a host-driven call): the whole control block (state
 // and per-block count slots) to zero.
-__global__ void k_state_reset(unsigned long long* words, unsigned long long n_words) {
+__global__ void k_state_reset(unsigned long long* words, unsigned long long n_words, unsigned long long stride) {
+    words = dev::job_ptr(words, dev::job_offset(stride));
     for(unsigned long long i = threadIdx.x; blockIdx.x == 0 && i < n_words; i += blockDim.x) words[i] = 0ull;
+}
+// The control blocks of a batch's jobs side by side in `out` (one copy to the host).
+__global__ void k_gather_states(const unsigned long long* st, unsigned long long stride, unsigned long long* out,
+                                unsigned words) {
+    st = dev::job_ptr(st, dev::job_offset(stride));
+    for(unsigned i = threadIdx.x; blockIdx.x == 0 && i < words; i += blockDim.x)
+        out[static_cast<unsigned long long>(blockIdx.y) * words + i] = st[i];
 }
 // Start of a round: pop K = min(k_max, S, room / (fanout - 1)) cuboids from
 // the top of the stack (their slots are reused by the children the round
 // pushes), and clear the round's list counters.
 __global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigned long long stack_cap,
-                              unsigned long long fanout) {
+                              unsigned long long fanout, unsigned long long stride) {
+    st = dev::job_ptr(st, dev::job_offset(stride));
     const unsigned t = threadIdx.x;
     if(blockIdx.x != 0) return;
     if(t == 0) {
@@ -113,7 +134,10 @@ __global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigne
 }
 // Adds the per-block count slots into the state's counters and clears them
 // (one block; run before the state is read).
-__global__ void k_fold_counts(DeviceState* st, unsigned long long* slots, unsigned long long n_slots) {
+__global__ void k_fold_counts(DeviceState* st, unsigned long long* slots, unsigned long long n_slots,
+                              unsigned long long stride) {
+    st = dev::job_ptr(st, dev::job_offset(stride));
+    slots = dev::job_ptr(slots, dev::job_offset(stride));
     constexpr int W = 5;   // used words of a slot
     __shared__ unsigned long long part[W][256];
     unsigned long long acc[W] = {0, 0, 0, 0, 0};
@@ -142,7 +166,8 @@ __global__ void k_levels_reset(DeviceState* st) {
     if(blockIdx.x == 0 && threadIdx.x < 32) st->undec_count[threadIdx.x] = 0;
 }
 // hull record initial values: min-key, max-key, zero-sign lb, zero-sign ub
-__global__ void k_init_hull(unsigned long long* p, unsigned long long n_records) {
+__global__ void k_init_hull(unsigned long long* p, unsigned long long n_records, unsigned long long stride) {
+    p = dev::job_ptr(p, dev::job_offset(stride));
     for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_records;
         i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
         p[4 * i + 0] = ~0ull;
@@ -168,7 +193,27 @@ template<typename T> T* dalloc(std::size_t n) {
     return static_cast<T*>(p);
 }
 
-// A device box list with its own storage; its counter lives in the DeviceState.
+// Device memory of one job, laid out before it is allocated: take() reserves an
+// aligned range and records where its address goes; place() sets every
+// recorded pointer once the allocation exists.
+struct Layout {
+    std::size_t size = 0;
+    std::vector<std::pair<std::size_t, std::function<void(char*)>>> fix;
+    template<typename T> void take(T*& p, std::size_t n) {
+        size = (size + 255) & ~static_cast<std::size_t>(255);
+        const std::size_t off = size;
+        T** where = &p;
+        fix.push_back({off, [where](char* at) { *where = reinterpret_cast<T*>(at); }});
+        size += (n ? n : 1) * sizeof(T);
+    }
+    std::size_t stride() const { return (size + 255) & ~static_cast<std::size_t>(255); }
+    void place(char* base) const {
+        for(const auto& f : fix) f.second(base + f.first);
+    }
+};
+
+// A device box list (storage in the core's job memory); its counter lives in
+// the DeviceState.
 struct DevList {
     int N = 0;
     unsigned long long cap = 0;
@@ -178,20 +223,13 @@ struct DevList {
     unsigned* origin = nullptr;
     unsigned* leaf = nullptr;
     unsigned long long* count = nullptr;
-    void create(int n_args, unsigned long long capacity, bool with_depth, bool with_origin, unsigned long long* counter) {
+    void lay(Layout& l, int n_args, unsigned long long capacity, bool with_depth, bool with_origin) {
         N = n_args;
         cap = capacity;
-        lb = dalloc<double>(static_cast<std::size_t>(cap) * N);
-        ub = dalloc<double>(static_cast<std::size_t>(cap) * N);
-        if(with_depth) depth = dalloc<unsigned>(cap);
-        if(with_origin) { origin = dalloc<unsigned>(cap); leaf = dalloc<unsigned>(cap); }
-        count = counter;
-    }
-    void destroy() {
-        for(void* p : {static_cast<void*>(lb), static_cast<void*>(ub), static_cast<void*>(depth),
-                       static_cast<void*>(origin), static_cast<void*>(leaf)})
-            if(p) (void)hipFree(p);
-        lb = ub = nullptr; depth = origin = leaf = nullptr; count = nullptr;
+        l.take(lb, static_cast<std::size_t>(cap) * N);
+        l.take(ub, static_cast<std::size_t>(cap) * N);
+        if(with_depth) l.take(depth, static_cast<std::size_t>(cap));
+        if(with_origin) { l.take(origin, static_cast<std::size_t>(cap)); l.take(leaf, static_cast<std::size_t>(cap)); }
     }
     dev::BoxList view() const { return view(count); }
     dev::BoxList view(unsigned long long* counter) const {
@@ -230,12 +268,20 @@ struct sqd_core {
     double* d_box_ct = nullptr;     // 2N: outward compile-time box
     double* d_box_init = nullptr;   // 2N: initial runtime box
     sqd::DeviceState* d_state = nullptr;   // followed by n_slots count slots of dev::kSlotWords words
-    sqd::DeviceState* h_state = nullptr;   // pinned host copy of the same block
+    sqd::DeviceState* h_state = nullptr;   // pinned host copies of the jobs' blocks (jobs of them)
+    // Batches: every pointer above and below marked (job) is job 0's; job j's
+    // memory is the same layout job_stride bytes further on, all in d_jobs.
+    unsigned jobs = 1;                      // jobs per batch (sqd_settings.jobs)
+    unsigned ny = 1;                        // grid rows of the launches being enqueued: 1 or jobs
+    unsigned states = 1;                    // jobs whose state the last copy to h_state holds
+    unsigned long long job_stride = 0;      // bytes of one job's part of d_jobs
+    char* d_jobs = nullptr;                 // the jobs' memory, jobs parts
+    unsigned long long* d_gather = nullptr; // jobs control blocks side by side (k_gather_states)
     unsigned long long* d_slots = nullptr;
     unsigned long long n_slots = 0;        // >= the largest grid of a counting kernel
     std::size_t state_bytes = 0;
     unsigned long long* d_hull = nullptr;
-    bool dirty = true;                // hull records and count slots need a full reset before use
+    unsigned clean = 0;               // jobs (from 0) whose hull records and count slots are known reset
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
     std::uint64_t device_bytes = 0;  // device memory allocated by sqd_core_create
@@ -339,29 +385,51 @@ void set_count(sqd_core* c, unsigned long long* p, unsigned long long v) {
 }
 void reset_state(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_state_reset", [&] {
-        hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(256), 0, c->stream, reinterpret_cast<unsigned long long*>(c->d_state),
-                           static_cast<unsigned long long>(c->state_bytes / sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_state_reset, dim3(1, c->ny), dim3(256), 0, c->stream, reinterpret_cast<unsigned long long*>(c->d_state),
+                           static_cast<unsigned long long>(c->state_bytes / sizeof(unsigned long long)), c->job_stride);
         return hipGetLastError();
     });
 }
 // Enqueues the fold of the count slots into the counters and the copy of the
-// device state to its pinned host copy.
+// device state (of every job of the launches' rows) to its pinned host copy.
 void enqueue_state_copy(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_fold_counts", [&] {
-        hipLaunchKernelGGL(k_fold_counts, dim3(1), dim3(256), 0, c->stream, c->d_state, c->d_slots, c->n_slots);
+        hipLaunchKernelGGL(k_fold_counts, dim3(1, c->ny), dim3(256), 0, c->stream, c->d_state, c->d_slots, c->n_slots,
+                           c->job_stride);
         return hipGetLastError();
     });
-    hip_check(hipMemcpyAsync(c->h_state, c->d_state, sizeof(DeviceState), hipMemcpyDeviceToHost, c->stream),
-              "copy state");
+    if(c->ny == 1) {
+        hip_check(hipMemcpyAsync(c->h_state, c->d_state, sizeof(DeviceState), hipMemcpyDeviceToHost, c->stream),
+                  "copy state");
+        return;
+    }
+    constexpr unsigned words = sizeof(DeviceState) / sizeof(unsigned long long);
+    launch(c, KIND_SET_COUNT, "k_gather_states", [&] {
+        hipLaunchKernelGGL(k_gather_states, dim3(1, c->ny), dim3(256), 0, c->stream,
+                           reinterpret_cast<const unsigned long long*>(c->d_state), c->job_stride, c->d_gather, words);
+        return hipGetLastError();
+    });
+    hip_check(hipMemcpyAsync(c->h_state, c->d_gather, sizeof(DeviceState) * c->ny, hipMemcpyDeviceToHost, c->stream),
+              "copy states");
 }
-// Waits for the state copy enqueued last and checks the overflow flags.
+// Waits for the state copy enqueued last and checks the overflow flags (of
+// every job: one failing job fails the whole call).
 const DeviceState& finish_state(sqd_core* c) {
     sync(c);
-    if(c->h_state->counters.overflow)
-        throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
-    if(c->h_state->frontier_full)
-        throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
+    c->states = c->ny;
+    for(unsigned j = 0; j < c->ny; ++j) {
+        if(c->h_state[j].counters.overflow)
+            throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
+        if(c->h_state[j].frontier_full)
+            throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
+    }
     return *c->h_state;
+}
+// Whether some job of the last state copy still has cuboids on its frontier.
+bool frontier_left(const sqd_core* c) {
+    for(unsigned j = 0; j < c->ny; ++j)
+        if(c->h_state[j].stack_count > 0) return true;
+    return false;
 }
 // Folds the count slots and brings the device state to the host (one copy, one sync).
 const DeviceState& read_state(sqd_core* c) {
@@ -394,6 +462,8 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.undec_cap = c->undec_cap;
     a.init_range = c->d_range;
     a.children_per_lane = static_cast<int>(std::min<uint32_t>(c->settings.prefilter_group, 64u));
+    a.job_stride = c->job_stride;
+    a.jobs = c->ny;
     return a;
 }
 
@@ -486,17 +556,19 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
 // with begin_call and marks the core clean when it completes; after a failed
 // call both are reset in full before the next use.
 void begin_call(sqd_core* c) {
-    if(c->dirty) {
+    if(c->clean < c->ny) {
         if(c->ks->NSTAT > 0)
             launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
-                hipLaunchKernelGGL(k_init_hull, dim3(256), dim3(256), 0, c->stream, c->d_hull,
-                                   c->k_max * static_cast<unsigned long long>(c->N));
+                hipLaunchKernelGGL(k_init_hull, dim3(256, c->ny), dim3(256), 0, c->stream, c->d_hull,
+                                   c->k_max * static_cast<unsigned long long>(c->N), c->job_stride);
                 return hipGetLastError();
             });
         reset_state(c);
     }
-    c->dirty = true;   // until the call completes
+    c->clean = 0;   // until the call completes
 }
+// A call that completed leaves the hull records and count slots of its jobs reset.
+void end_call(sqd_core* c) { c->clean = c->ny; }
 void levels_reset(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_levels_reset", [&] {
         hipLaunchKernelGGL(k_levels_reset, dim3(1), dim3(64), 0, c->stream, c->d_state);
@@ -514,17 +586,23 @@ void launch_initial_box(sqd_core* c, bool to_scratch, bool reset) {
     const unsigned words = reset ? static_cast<unsigned>(sizeof(DeviceState) / sizeof(unsigned long long)) : 0u;
     launch(c, KIND_INITIAL_BOX, "initial box", [&] {
         return c->ks->initial_box(c->d_box_ct, c->d_box_ct + c->N, c->d_consts, c->d_box_init, list, zero, words,
-                                  c->stream);
+                                  c->ny, c->job_stride, c->stream);
     });
 }
 
-// The range of the first dynamic variable's initial subdivisions that the next
-// initial queue starts from.
-void set_range(sqd_core* c, unsigned long long first, unsigned long long last) {
-    launch(c, KIND_SET_COUNT, "k_set_range", [&] {
-        hipLaunchKernelGGL(k_set_range, dim3(1), dim3(64), 0, c->stream, c->d_range, first, last);
+// The ranges of the first dynamic variable's initial subdivisions that the next
+// initial queues (one per job of the launches' rows) start from.
+void set_ranges(sqd_core* c, const JobRanges& r) {
+    launch(c, KIND_SET_COUNT, "k_set_ranges", [&] {
+        hipLaunchKernelGGL(k_set_ranges, dim3(1, c->ny), dim3(64), 0, c->stream, c->d_range, c->job_stride, r);
         return hipGetLastError();
     });
+}
+void set_range(sqd_core* c, unsigned long long first, unsigned long long last) {
+    JobRanges r{};
+    r.lo[0] = static_cast<unsigned>(first);
+    r.hi[0] = static_cast<unsigned>(last);
+    set_ranges(c, r);
 }
 unsigned long long full_range(const sqd_core* c) { return c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]; }
 
@@ -592,8 +670,8 @@ template<typename F> hipGraphExec_t capture_graph(sqd_core* c, F&& enqueue) {
 void enqueue_round(sqd_core* c) {
     DeviceState* st = c->d_state;
     launch(c, KIND_SET_COUNT, "k_round_begin", [&] {
-        hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, c->stream, st, c->k_max, c->stack.cap,
-                           c->split_fanout);
+        hipLaunchKernelGGL(k_round_begin, dim3(1, c->ny), dim3(64), 0, c->stream, st, c->k_max, c->stack.cap,
+                           c->split_fanout, c->job_stride);
         return hipGetLastError();
     });
     const unsigned long long K = c->k_max;   // bound on the round's size
@@ -601,14 +679,85 @@ void enqueue_round(sqd_core* c) {
     if(c->ks->NSTAT > 0) run_static(c, dev::NONFINAL, items, &st->round_base, &st->round_k, K, c->crit.view());
     launch(c, KIND_HULL_FINALIZE, "hull finalize", [&] {
         return c->ks->hull_finalize(items, &st->round_base, &st->round_k, K, c->d_hull, c->fin.view(), c->spl.view(),
-                                    &st->counters, c->d_slots, c->settings.generation_count, c->stream);
+                                    &st->counters, c->d_slots, c->settings.generation_count, c->ny, c->job_stride,
+                                    c->stream);
     });
     if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, K, c->crit.view());
     else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
         return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_slots,
-                                     static_cast<unsigned>((K + 255) / 256), c->stream);
+                                     static_cast<unsigned>((K + 255) / 256), c->ny, c->job_stride, c->stream);
     });
     run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, K, c->stack.view());
+}
+
+// Proves the initial ranges [first[j], last[j]) of n jobs, leaving each job's
+// final state in h_state[j].  A core of one job runs one grid row; a batch core
+// always runs all its rows (the jobs beyond n get empty ranges and do no work),
+// so that its captured graphs fit every call.  One host round trip per round:
+// the round's size is decided on the device, the host only learns whether some
+// frontier is left.  From the second proof on a core (the first one fills the
+// launch statistics), the rounds are replayed as captured graphs unless
+// profiling is on.  Progress reports (single jobs only) follow job 0.
+void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* last, double* seconds,
+                bool progress_reports) {
+    const uint32_t all = c->dyn_initial.empty() ? 1u : static_cast<uint32_t>(c->dyn_initial[0]);
+    JobRanges r{};
+    for(unsigned j = 0; j < n; ++j) {
+        if(first[j] >= last[j] || last[j] > all)
+            throw std::invalid_argument("initial range must satisfy first < last <= the first variable's subdivisions");
+        r.lo[j] = first[j];
+        r.hi[j] = last[j];
+    }
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    auto t0 = std::chrono::steady_clock::now();
+    struct Rows {           // every launch of the call runs the core's rows
+        sqd_core* c;
+        explicit Rows(sqd_core* core) : c(core) { c->ny = c->jobs; }
+        ~Rows() { c->ny = 1; }
+    } rows(c);
+    set_ranges(c, r);
+    begin_call(c);
+    const bool graphs = c->settings.graphs && !c->profiling && c->proofs > 0;
+    if(graphs && !c->g_first) {
+        c->g_first = capture_graph(c, [&] { enqueue_initial_queue(c); enqueue_round(c); enqueue_state_copy(c); });
+        c->g_round = capture_graph(c, [&] { enqueue_round(c); enqueue_state_copy(c); });
+    }
+    unsigned long long rounds = 0;
+    auto last_report = std::chrono::steady_clock::now();
+    bool reported = false;
+    auto progress = [&] {                      // between rounds, when due
+        if(!progress_reports || !c->progress_fn) return;
+        const auto now = std::chrono::steady_clock::now();
+        if(std::chrono::duration<double>(now - t0).count() < c->progress_start) return;
+        if(reported && std::chrono::duration<double>(now - last_report).count() < c->progress_every) return;
+        reported = true;
+        last_report = now;
+        report_progress(c, rounds, false, SQD_OK);
+    };
+    if(graphs) {
+        hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
+        finish_state(c);
+        ++rounds;
+        while(frontier_left(c)) {
+            progress();
+            hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
+            finish_state(c);
+            ++rounds;
+        }
+    } else {
+        enqueue_initial_queue(c);
+        do {
+            if(rounds > 0) progress();
+            enqueue_round(c);
+            read_state(c);
+            ++rounds;
+        } while(frontier_left(c));
+    }
+    ++c->proofs;
+    end_call(c);
+    if(progress_reports) report_progress(c, rounds, true, SQD_OK);
+    auto t1 = std::chrono::steady_clock::now();
+    if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
 } // namespace
@@ -745,38 +894,59 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->n_slots = std::max<unsigned long long>(8ull * c->cus, kmax / 256 + 2);
         if(c->grid_override > c->n_slots) c->grid_override = static_cast<unsigned>(c->n_slots);
         c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * dev::kSlotWords * sizeof(unsigned long long);
-        c->d_state = static_cast<DeviceState*>(static_cast<void*>(dalloc<unsigned long long>(c->state_bytes / 8)));
-        c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
-        void* hs = nullptr;
-        hip_check(hipHostMalloc(&hs, c->state_bytes, hipHostMallocDefault), "hipHostMalloc");
-        c->h_state = static_cast<DeviceState*>(hs);
-        DeviceState* st = c->d_state;
-        c->stack.create(c->N, std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1), true, false,
-                        &st->stack_count);
-        for(auto& l : c->lvl) l.create(c->N, lvl_cap, true, true, nullptr);
-        c->fin.create(c->N, kmax, true, false, &st->fin_count);
-        c->spl.create(c->N, kmax, true, false, &st->spl_count);
-        c->crit.create(c->N, s.critical_capacity, false, false, &st->crit_count);
-        c->scratch.create(c->N, std::max<unsigned long long>(kmax, 1024), true, false, &st->scratch_count);
-        c->d_hull = dalloc<unsigned long long>(static_cast<std::size_t>(kmax) * c->N * 4);
-        c->d_range = dalloc<unsigned long long>(2);
-        const unsigned long long full[2] = {0ull, c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]};
-        hip_check(hipMemcpyAsync(c->d_range, full, sizeof full, hipMemcpyHostToDevice, c->stream), "initial range");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        // one job's memory, laid out once; a batch holds `jobs` copies of it
+        Layout lay;
+        unsigned long long* state_words = nullptr;
+        lay.take(state_words, c->state_bytes / sizeof(unsigned long long));
+        const unsigned long long stack_cap = std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1);
+        c->stack.lay(lay, c->N, stack_cap, true, false);
+        for(auto& l : c->lvl) l.lay(lay, c->N, lvl_cap, true, true);
+        c->fin.lay(lay, c->N, kmax, true, false);
+        c->spl.lay(lay, c->N, kmax, true, false);
+        c->crit.lay(lay, c->N, s.critical_capacity, false, false);
+        c->scratch.lay(lay, c->N, std::max<unsigned long long>(kmax, 1024), true, false);
+        lay.take(c->d_hull, static_cast<std::size_t>(kmax) * c->N * 4);
+        lay.take(c->d_range, 2);
         if(c->fp32) {   // as many undecided children as a level buffer holds boxes
             c->undec_cap = lvl_cap;
-            c->d_undec = dalloc<unsigned long long>(static_cast<std::size_t>(lvl_cap));
+            lay.take(c->d_undec, static_cast<std::size_t>(lvl_cap));
         }
-        c->d_has = dalloc<unsigned char>(static_cast<std::size_t>(kmax));
-        c->d_consts = dalloc<double>(2 * static_cast<std::size_t>(std::max(1, c->ks->NC)));
-        hip_check(hipMemcpyAsync(c->d_consts, c->ks->consts, 2 * sizeof(double) * static_cast<std::size_t>(std::max(1, c->ks->NC)),
-                                 hipMemcpyHostToDevice, c->stream), "upload constants");
-        c->d_box_ct = dalloc<double>(2 * static_cast<std::size_t>(c->N));
-        c->d_box_init = dalloc<double>(2 * static_cast<std::size_t>(c->N));
+        lay.take(c->d_has, static_cast<std::size_t>(kmax));
+        const std::size_t n_consts = 2 * static_cast<std::size_t>(std::max(1, c->ks->NC));
+        lay.take(c->d_consts, n_consts);
+        lay.take(c->d_box_ct, 2 * static_cast<std::size_t>(c->N));
+        lay.take(c->d_box_init, 2 * static_cast<std::size_t>(c->N));
+        c->jobs = s.jobs ? s.jobs : 1u;
+        if(c->jobs > kMaxJobs) throw std::invalid_argument("sqd_settings.jobs: at most 64 jobs per batch");
+        c->job_stride = lay.stride();
+        c->d_jobs = dalloc<char>(static_cast<std::size_t>(c->job_stride) * c->jobs);
+        lay.place(c->d_jobs);
+        c->d_state = reinterpret_cast<DeviceState*>(state_words);
+        c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
+        DeviceState* st = c->d_state;
+        c->stack.count = &st->stack_count;
+        c->fin.count = &st->fin_count;
+        c->spl.count = &st->spl_count;
+        c->crit.count = &st->crit_count;
+        c->scratch.count = &st->scratch_count;
+        if(c->jobs > 1) c->d_gather = dalloc<unsigned long long>(sizeof(DeviceState) / sizeof(unsigned long long) * c->jobs);
+        void* hs = nullptr;
+        hip_check(hipHostMalloc(&hs, sizeof(DeviceState) * c->jobs, hipHostMallocDefault), "hipHostMalloc");
+        c->h_state = static_cast<DeviceState*>(hs);
+        std::memset(c->h_state, 0, sizeof(DeviceState) * c->jobs);
+        // every job's copy of the constants, the compile-time box and the full range
+        const unsigned long long full[2] = {0ull, c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]};
         std::vector<double> box(prog.box_lb);
         box.insert(box.end(), prog.box_ub.begin(), prog.box_ub.end());
-        hip_check(hipMemcpyAsync(c->d_box_ct, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
-                  "upload box");
+        for(unsigned j = 0; j < c->jobs; ++j) {
+            const std::size_t off = static_cast<std::size_t>(c->job_stride) * j;
+            auto at = [&](auto* q) { return reinterpret_cast<char*>(q) + off; };
+            hip_check(hipMemcpyAsync(at(c->d_range), full, sizeof full, hipMemcpyHostToDevice, c->stream), "initial range");
+            hip_check(hipMemcpyAsync(at(c->d_consts), c->ks->consts, sizeof(double) * n_consts, hipMemcpyHostToDevice,
+                                     c->stream), "upload constants");
+            hip_check(hipMemcpyAsync(at(c->d_box_ct), box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     c->stream), "upload box");
+        }
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->device_bytes = t_dalloc_bytes - bytes0;
         *out = c.release();
@@ -788,12 +958,7 @@ void sqd_core_destroy(sqd_core* c) {
     std::lock_guard<std::mutex> lock(g_capture_mutex);
     (void)hipSetDevice(c->device);
     if(c->stream) (void)hipStreamSynchronize(c->stream);
-    c->stack.destroy();
-    for(auto& l : c->lvl) l.destroy();
-    c->fin.destroy(); c->spl.destroy(); c->crit.destroy(); c->scratch.destroy();
-    for(void* p : {static_cast<void*>(c->d_hull), static_cast<void*>(c->d_has), static_cast<void*>(c->d_state),
-                   static_cast<void*>(c->d_undec), static_cast<void*>(c->d_range),
-                   static_cast<void*>(c->d_consts), static_cast<void*>(c->d_box_ct), static_cast<void*>(c->d_box_init)})
+    for(void* p : {static_cast<void*>(c->d_jobs), static_cast<void*>(c->d_gather)})
         if(p) (void)hipFree(p);
     if(c->h_state) (void)hipHostFree(c->h_state);
     if(c->g_first) (void)hipGraphExecDestroy(c->g_first);
@@ -825,7 +990,7 @@ int sqd_initial_queue(sqd_core* c, double* lb, double* ub, size_t capacity, size
         const DeviceState& h = read_state(c);
         const unsigned long long len = h.stack_count;
         const sqd_counts r = counts_of(h);
-        c->dirty = false;
+        end_call(c);
         *n = len;
         if(lb && ub) download_soa(c, c->stack, std::min<unsigned long long>(len, capacity), lb, ub, capacity);
         if(counts) *counts = r;
@@ -867,7 +1032,7 @@ int sqd_nonfinal(sqd_core* c, const double* lb, const double* ub, size_t n,
                 }
         }
         sqd_counts r = counts_of(read_state(c));
-        c->dirty = false;
+        end_call(c);
         if(c->ks->NSTAT == 0) r.num_leaf_cuboids += n;  // without static variables each box is its own leaf
         if(counts) *counts = r;
     });
@@ -885,12 +1050,12 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
         if(c->ks->NSTAT > 0) run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, n, c->crit.view());
         else launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
             return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_slots,
-                                         static_cast<unsigned>((n + 255) / 256 + 1), c->stream);
+                                         static_cast<unsigned>((n + 255) / 256 + 1), 1u, 0ull, c->stream);
         });
         const DeviceState& h = read_state(c);
         const unsigned long long k = h.crit_count;
         const sqd_counts r = counts_of(h);
-        c->dirty = false;
+        end_call(c);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
         if(counts) *counts = r;
@@ -910,7 +1075,7 @@ int sqd_split(sqd_core* c, const double* lb, const double* ub, size_t n,
         const DeviceState& h = read_state(c);
         const unsigned long long k = h.stack_count;
         const sqd_counts r = counts_of(h);
-        c->dirty = false;
+        end_call(c);
         if(n_out) *n_out = k;
         if(out_lb && out_ub) download_soa(c, c->stack, std::min<unsigned long long>(k, capacity), out_lb, out_ub, capacity);
         if(counts) *counts = r;
@@ -928,62 +1093,10 @@ int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* coun
                     double* crit_ub, size_t capacity, size_t* n_critical, double* seconds) {
     const int rc = guarded([&] {
         if(!c) throw std::invalid_argument("null core");
-        const uint32_t all = c->dyn_initial.empty() ? 1u : static_cast<uint32_t>(c->dyn_initial[0]);
-        if(first >= last || last > all)
-            throw std::invalid_argument("initial range must satisfy first < last <= the first variable's subdivisions");
-        hip_check(hipSetDevice(c->device), "hipSetDevice");
-        auto t0 = std::chrono::steady_clock::now();
-        set_range(c, first, last);
-        begin_call(c);
-        // One host round trip per round: the round's size is decided on the
-        // device, the host only learns whether the frontier is empty.  From the
-        // second proof on a core (the first one fills the launch statistics),
-        // the rounds are replayed as captured graphs unless profiling is on.
-        const bool graphs = c->settings.graphs && !c->profiling && c->proofs > 0;
-        if(graphs && !c->g_first) {
-            c->g_first = capture_graph(c, [&] { enqueue_initial_queue(c); enqueue_round(c); enqueue_state_copy(c); });
-            c->g_round = capture_graph(c, [&] { enqueue_round(c); enqueue_state_copy(c); });
-        }
-        const DeviceState* h = nullptr;
-        unsigned long long rounds = 0;
-        auto last_report = std::chrono::steady_clock::now();
-        bool reported = false;
-        auto progress = [&] {                      // between rounds, when due
-            if(!c->progress_fn) return;
-            const auto now = std::chrono::steady_clock::now();
-            if(std::chrono::duration<double>(now - t0).count() < c->progress_start) return;
-            if(reported && std::chrono::duration<double>(now - last_report).count() < c->progress_every) return;
-            reported = true;
-            last_report = now;
-            report_progress(c, rounds, false, SQD_OK);
-        };
-        if(graphs) {
-            hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
-            h = &finish_state(c);
-            ++rounds;
-            while(h->stack_count > 0) {
-                progress();
-                hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
-                h = &finish_state(c);
-                ++rounds;
-            }
-        } else {
-            enqueue_initial_queue(c);
-            do {
-                if(rounds > 0) progress();
-                enqueue_round(c);
-                h = &read_state(c);
-                ++rounds;
-            } while(h->stack_count > 0);
-        }
-        ++c->proofs;
-        c->dirty = false;
-        report_progress(c, rounds, true, SQD_OK);
-        const sqd_counts r = counts_of(*h);
-        const unsigned long long k = h->crit_count;
-        auto t1 = std::chrono::steady_clock::now();
-        if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-        if(counts) *counts = r;
+        prove_jobs(c, 1, &first, &last, seconds, true);
+        const DeviceState& h = c->h_state[0];
+        const unsigned long long k = h.crit_count;
+        if(counts) *counts = counts_of(h);
         if(n_critical) *n_critical = k;
         if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
     });
@@ -993,6 +1106,40 @@ int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* coun
         t_error = msg;
     }
     return rc;
+}
+
+int sqd_prove_batch(sqd_core* c, size_t n_jobs, const uint32_t* first, const uint32_t* last,
+                    sqd_job_counts* results, double* seconds) {
+    return guarded([&] {
+        if(!c || !first || !last || !results) throw std::invalid_argument("null argument");
+        if(n_jobs < 1 || n_jobs > c->jobs) throw std::invalid_argument("a batch holds 1 to sqd_settings.jobs jobs");
+        prove_jobs(c, static_cast<unsigned>(n_jobs), first, last, seconds, false);
+        for(size_t j = 0; j < n_jobs; ++j) {
+            results[j].counts = counts_of(c->h_state[j]);
+            results[j].n_critical = c->h_state[j].crit_count;
+            results[j].fp32_checked = c->h_state[j].f32_checked;
+            results[j].fp32_decided = c->h_state[j].f32_decided;
+        }
+    });
+}
+
+int sqd_core_job_criticals(sqd_core* c, size_t job, double* crit_lb, double* crit_ub, size_t capacity, size_t* n) {
+    return guarded([&] {
+        if(!c || !n) throw std::invalid_argument("null argument");
+        if(job >= c->jobs) throw std::invalid_argument("job index beyond the core's batch size");
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        // the last call's state copy holds the job's count when the call was a batch
+        const unsigned long long k = job < c->states ? c->h_state[job].crit_count : 0ull;
+        *n = k;
+        if(crit_lb && crit_ub && k) {
+            DevList l = c->crit;
+            const std::size_t off = static_cast<std::size_t>(c->job_stride) * job;
+            l.lb = reinterpret_cast<double*>(reinterpret_cast<char*>(l.lb) + off);
+            l.ub = reinterpret_cast<double*>(reinterpret_cast<char*>(l.ub) + off);
+            download_soa(c, l, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+        }
+    });
 }
 
 int sqd_core_set_progress(sqd_core* c, sqd_progress_fn fn, void* user, double start_after_seconds,

@@ -303,6 +303,16 @@ struct sqd_service {
             const uint64_t id = queue.front();
             queue.pop_front();
             const sqd_program* prog = jobs[id].prog;
+            // the batch: this job and the next queued ones of the same program
+            std::vector<uint64_t> batch{id};
+            for(auto it = queue.begin(); it != queue.end() && batch.size() < opt.batch_jobs;) {
+                if(jobs[*it].prog == prog) {
+                    batch.push_back(*it);
+                    it = queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
             // settings of the program (tuned once, by the first worker that needs them)
             if(!settings.count(prog)) {
                 if(opt.autotune && !tuning[prog]) {
@@ -325,9 +335,16 @@ struct sqd_service {
             }
             const uint64_t slot = acquire(lk, prog);
             if(slot == 0) return;
-            jobs[id].started = Clock::now();
-            const uint32_t first = jobs[id].first, last = jobs[id].last;
-            const sqd_settings s = settings[prog];
+            const size_t n = batch.size();
+            std::vector<uint32_t> first(n), last(n);
+            const Clock::time_point started = Clock::now();
+            for(size_t i = 0; i < n; ++i) {
+                jobs[batch[i]].started = started;
+                first[i] = jobs[batch[i]].first;
+                last[i] = jobs[batch[i]].last;
+            }
+            sqd_settings s = settings[prog];
+            s.jobs = opt.batch_jobs;
             sqd_core* core = find_slot(slot)->core;
             lk.unlock();
             int rc = SQD_OK;
@@ -336,12 +353,17 @@ struct sqd_service {
                 rc = sqd_core_create(device, prog, &s, &core);
                 if(rc != SQD_OK) err = sqd_last_error();
             }
-            sqd_job_result r{};
+            std::vector<sqd_job_counts> jc(n);
+            std::vector<int> status(n, rc);
+            std::vector<std::string> errors(n, err);
             if(rc == SQD_OK) {
-                size_t nc = 0;
-                rc = sqd_prove_range(core, first, last, &r.counts, nullptr, nullptr, 0, &nc, nullptr);
-                if(rc != SQD_OK) err = sqd_last_error();
-                r.n_critical = nc;
+                if(sqd_prove_batch(core, n, first.data(), last.data(), jc.data(), nullptr) != SQD_OK) {
+                    // a failing job fails its batch: prove the jobs one by one to tell which
+                    for(size_t i = 0; i < n; ++i) {
+                        status[i] = sqd_prove_batch(core, 1, &first[i], &last[i], &jc[i], nullptr);
+                        if(status[i] != SQD_OK) errors[i] = sqd_last_error();
+                    }
+                }
             }
             uint64_t bytes = 0;
             if(core) sqd_core_memory(core, &bytes);
@@ -359,15 +381,21 @@ struct sqd_service {
                 bytes_in_use -= sl->bytes;
                 pool.erase(pool.begin() + (sl - pool.data()));
             }
-            Job& j = jobs[id];
-            j.finished = Clock::now();
-            r.status = rc;
-            r.queued_seconds = seconds_since(j.submitted, j.started);
-            r.run_seconds = seconds_since(j.started, j.finished);
-            r.latency_seconds = seconds_since(j.submitted, j.finished);
-            j.result = r;
-            j.error = err;
-            j.done = true;
+            const Clock::time_point finished = Clock::now();
+            for(size_t i = 0; i < n; ++i) {
+                Job& j = jobs[batch[i]];
+                j.finished = finished;
+                sqd_job_result r{};
+                r.counts = jc[i].counts;
+                r.n_critical = jc[i].n_critical;
+                r.status = status[i];
+                r.queued_seconds = seconds_since(j.submitted, j.started);
+                r.run_seconds = seconds_since(j.started, j.finished);
+                r.latency_seconds = seconds_since(j.submitted, j.finished);
+                j.result = r;
+                j.error = errors[i];
+                j.done = true;
+            }
             cv.notify_all();
         }
     }
@@ -380,12 +408,13 @@ void sqd_service_options_default(sqd_service_options* o) {
     *o = sqd_service_options{};
     o->device = 0;
     o->workers = 8;
-    o->memory_budget = 64ull << 30;
+    o->memory_budget = 128ull << 30;
     o->fp32_prefilter = SQD_FP32_ALL_LEVELS;
     o->autotune = 0;
     o->level_bytes = 256ull << 20;
     o->frontier_capacity = 1ull << 22;
     o->core_bytes_estimate = 2ull << 30;
+    o->batch_jobs = 8;
 }
 
 int sqd_service_create(const sqd_service_options* opt, sqd_service** out) {
@@ -397,6 +426,8 @@ int sqd_service_create(const sqd_service_options* opt, sqd_service** out) {
         if(s->opt.workers < 1) s->opt.workers = 1;
         if(s->opt.workers > 64) s->opt.workers = 64;
         if(s->opt.core_bytes_estimate == 0) s->opt.core_bytes_estimate = 2ull << 30;
+        if(s->opt.batch_jobs < 1) s->opt.batch_jobs = 1;
+        if(s->opt.batch_jobs > 64) s->opt.batch_jobs = 64;
         const int device = s->opt.device;
         for(uint32_t i = 0; i < s->opt.workers; ++i) s->workers.emplace_back([s, device] { s->run_worker(device); });
         *out = s;

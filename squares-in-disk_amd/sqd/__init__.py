@@ -37,7 +37,8 @@ class Settings(ctypes.Structure):
     _fields_ = [("generation_count", ctypes.c_uint32), ("dequeue_buffer_size", ctypes.c_uint32),
                 ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
                 ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
-                ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("prefilter_group", ctypes.c_uint32)]
+                ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("prefilter_group", ctypes.c_uint32),
+                ("jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 FP32_ALL_LEVELS = 0xFFFFFFFF
@@ -61,7 +62,13 @@ class AutotuneReport(ctypes.Structure):
 class ServiceOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("workers", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64),
                 ("fp32_prefilter", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("level_bytes", ctypes.c_uint64),
-                ("frontier_capacity", ctypes.c_uint64), ("core_bytes_estimate", ctypes.c_uint64)]
+                ("frontier_capacity", ctypes.c_uint64), ("core_bytes_estimate", ctypes.c_uint64),
+                ("batch_jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class JobCounts(ctypes.Structure):
+    _fields_ = [("counts", Counts), ("n_critical", ctypes.c_uint64), ("fp32_checked", ctypes.c_uint64),
+                ("fp32_decided", ctypes.c_uint64)]
 
 
 class JobResult(ctypes.Structure):
@@ -129,6 +136,9 @@ def lib():
             "sqd_core_criticals": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(_SZ)]),
             "sqd_prove_range": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Counts), _D, _D, _SZ,
                                                ctypes.POINTER(_SZ), ctypes.POINTER(ctypes.c_double)]),
+            "sqd_prove_batch": (ctypes.c_int, [_P, _SZ, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.POINTER(JobCounts), ctypes.POINTER(ctypes.c_double)]),
+            "sqd_core_job_criticals": (ctypes.c_int, [_P, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ)]),
             "sqd_core_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
             "sqd_core_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_autotune_options_default": (None, [ctypes.POINTER(AutotuneOptions)]),
@@ -165,6 +175,7 @@ def exported_symbols():
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_prove_range",
+            "sqd_prove_batch", "sqd_core_job_criticals",
             "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
             "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
             "sqd_service_wait", "sqd_service_destroy",
@@ -339,6 +350,32 @@ class Core:
         _check(lib().sqd_prove_range(self._h, first, last, ctypes.byref(c), None, None, 0, ctypes.byref(k),
                                      ctypes.byref(secs)))
         return {"counts": c.as_dict(), "n_critical": int(k.value), "seconds": secs.value}
+
+    def prove_batch(self, ranges, critical_capacity=0):
+        """Jobs [(first, last), ...] of this program in the same launches (at most
+        the core's settings.jobs); one dict per job as prove_range gives, plus the
+        job's criticals (up to critical_capacity) and FP32 prefilter counts."""
+        n = len(ranges)
+        first = (ctypes.c_uint32 * max(n, 1))(*[int(a) for a, _ in ranges])
+        last = (ctypes.c_uint32 * max(n, 1))(*[int(b) for _, b in ranges])
+        res = (JobCounts * max(n, 1))()
+        secs = ctypes.c_double()
+        _check(lib().sqd_prove_batch(self._h, n, first, last, res, ctypes.byref(secs)))
+        out = []
+        for j in range(n):
+            r = res[j]
+            d = {"counts": r.counts.as_dict(), "n_critical": int(r.n_critical), "seconds": secs.value,
+                 "fp32_stats": (int(r.fp32_checked), int(r.fp32_decided))}
+            m = min(int(r.n_critical), critical_capacity)
+            if critical_capacity:
+                cl = np.zeros((self.N, m))
+                cu = np.zeros((self.N, m))
+                if m:
+                    k = _SZ()
+                    _check(lib().sqd_core_job_criticals(self._h, j, _dptr(cl), _dptr(cu), m, ctypes.byref(k)))
+                d["crit_lb"], d["crit_ub"] = cl, cu
+            out.append(d)
+        return out
 
     def set_progress(self, fn, start_after=0.0, every=0.0):
         """fn(dict) after each due round and at the end of sqd_prove (None: off)."""
