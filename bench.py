@@ -22,8 +22,10 @@ checked against the published ones.  The line printed carries
               same proof
   fp32_first_batched
               the same proofs through the proof service: the program's settings
-              autotuned (FP32 prefilter levels, cuboids per round, blocks per
-              launch), `--workers` proofs in flight on their own streams; its
+              autotuned on batches (FP32 prefilter levels and children per lane,
+              cuboids per round, blocks per launch), `--workers` workers each
+              proving up to `--batch-jobs` queued proofs in the same launches
+              (one grid row per proof) on its own stream; its
               cuboids/s, p50/p99 job latency and ratio to `value`, the fraction
               of static-level children the FP32 pass decided, and the
               FP32-versus-FP64 width ratio of each constraint on sampled boxes
@@ -34,7 +36,7 @@ checked against the published ones.  The line printed carries
               it: p50/p99 latency and cuboids/s; every job's counts are checked
               against the sum of its subtrees' (a sequential binary64 run)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one] [--workers 8]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one] [--workers 4] [--batch-jobs 8]
 """
 import argparse
 import json
@@ -75,8 +77,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=None,
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
-    ap.add_argument("--workers", type=int, default=8, help="proofs in flight in the batched legs")
-    ap.add_argument("--batched-proofs", type=int, default=256)
+    ap.add_argument("--workers", type=int, default=4, help="batches in flight in the batched legs")
+    ap.add_argument("--batch-jobs", type=int, default=8, help="proofs per batch (one grid row each)")
+    ap.add_argument("--batched-proofs", type=int, default=1024)
     ap.add_argument("--stream-jobs", type=int, default=48)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-batched", action="store_true")
@@ -180,13 +183,20 @@ def initial_subdivisions(program):
     return 1
 
 
-def fp32_first_batched(sqd, prog, expected, workers, proofs, plain_value, counted32):
+def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_value, counted32):
     """The batched FP32-first leg on `prog` (full proofs through the service)."""
-    best, tune = sqd.autotune(prog)
+    import ctypes
+    svc_opts = sqd.ServiceOptions()
+    sqd.lib().sqd_service_options_default(ctypes.byref(svc_opts))
+    base = sqd.default_settings(jobs=batch_jobs, level_bytes=svc_opts.level_bytes,
+                                frontier_capacity=svc_opts.frontier_capacity)
+    best, tune = sqd.autotune(prog, base=base)            # as the service tunes it (reported)
     tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "prefilter_group", "dequeue_buffer_size", "grid_blocks")}
     n = initial_subdivisions(prog)
     # decided fraction and the dominant prefiltered kernel's time on one core
-    with sqd.Core(prog, settings=best) as core:
+    one = sqd.Settings.from_buffer_copy(best)
+    one.jobs = 1
+    with sqd.Core(prog, settings=one) as core:
         core.prove()
         core.set_profiling(True)
         for _ in range(20):
@@ -194,9 +204,12 @@ def fp32_first_batched(sqd, prog, expected, workers, proofs, plain_value, counte
         ks = core.kernel_stats()
         checked, decided = core.fp32_stats()
         core.set_profiling(False)
-    with sqd.Service(workers=workers, autotune=1) as svc:
-        for _ in range(2 * workers):                          # tunes once, then warms every worker's core
-            svc.wait(svc.submit(prog, 0, n))
+    with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
+        svc.set_settings(prog, best)                          # the settings reported above
+        for _ in range(2):                                    # creates and warms every worker's core
+            warm = [svc.submit(prog, 0, n) for _ in range(workers * batch_jobs)]
+            for i in warm:
+                svc.wait(i)
         t0 = time.perf_counter()
         ids = [svc.submit(prog, 0, n) for _ in range(proofs)]
         res = [svc.wait(i) for i in ids]
@@ -209,7 +222,7 @@ def fp32_first_batched(sqd, prog, expected, workers, proofs, plain_value, counte
     value = sum(r["counts"]["num_cuboids"] for r in res) / elapsed
     out = {
         "value": value, "unit": "cuboids/s", "ratio_vs_plain_fp64": value / plain_value,
-        "workers": workers, "proofs": proofs, "seconds": elapsed,
+        "workers": workers, "batch_jobs": batch_jobs, "proofs": proofs, "seconds": elapsed,
         "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
         "run_ms": {"p50": percentile([r["run_seconds"] for r in res], 0.5) * 1e3},
         "autotuned": tuned, "autotune_slice": [tune["first"], tune["last"]],
@@ -243,7 +256,7 @@ def width_sample(sqd, prog, n_boxes=8192, seed=3):
                                        "samples": s["samples"]} for s in st if s["samples"]}
 
 
-def job_stream(sqd, jobs_n, workers, seed):
+def job_stream(sqd, jobs_n, workers, batch_jobs, seed):
     """Poisson stream of proof jobs over all statements (see the module docstring)."""
     rng = random.Random(seed)
     progs = {k: sqd.Program.from_lemma(k) for k in LEMMAS}
@@ -260,7 +273,7 @@ def job_stream(sqd, jobs_n, workers, seed):
     with sqd.Service(workers=1, fp32_prefilter=0) as svc:
         for j in set(jobs):
             alone[j] = svc.wait(svc.submit(progs[j[0]], j[1], j[2]))["counts"]
-    with sqd.Service(workers=workers, autotune=1) as svc:
+    with sqd.Service(workers=workers, autotune=1, batch_jobs=batch_jobs) as svc:
         for k in LEMMAS:                                      # tune and create a core per statement
             svc.wait(svc.submit(progs[k], 0, 1))
         t0 = time.perf_counter()
@@ -287,7 +300,7 @@ def job_stream(sqd, jobs_n, workers, seed):
     lat = [r["latency_seconds"] for r in res]
     cub = sum(r["counts"]["num_cuboids"] for r in res)
     return {
-        "jobs": len(jobs), "seed": seed, "workers": workers, "arrival_rate_per_s": rate,
+        "jobs": len(jobs), "seed": seed, "workers": workers, "batch_jobs": batch_jobs, "arrival_rate_per_s": rate,
         "capacity_jobs_per_s": len(jobs) / makespan,
         "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
         "cuboids_per_s": cub / elapsed, "cuboids": cub, "seconds": elapsed,
@@ -341,11 +354,11 @@ def main():
         counted, cpu = cpu_baseline(prog.text(), a.cpu_runs)
         counted32 = fp32_op_counts(prog.text())
     value = total_cuboids / elapsed
-    batched = None if a.no_batched else fp32_first_batched(sqd, prog, expected, a.workers, a.batched_proofs, value,
-                                                           counted32)
+    batched = None if a.no_batched else fp32_first_batched(sqd, prog, expected, a.workers, a.batch_jobs,
+                                                           a.batched_proofs, value, counted32)
     if batched is not None:
         batched["width_ratio_per_constraint"] = width_sample(sqd, prog)
-    stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.seed)
+    stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.batch_jobs, a.seed)
 
     dom = next(s for s in stats if s["mode"] == dom["mode"] and s["level"] == dom["level"])
     mean_ms = dom["milliseconds"] / dom["launches"]

@@ -240,12 +240,14 @@ int sqd_core_shape(const sqd_core* core, uint32_t* initial_subdivisions, uint32_
 
 /* ---------------------------------------------------------------- autotuner */
 
-/* Chooses, by coordinate descent over timed sqd_prove_range runs on a slice of
- * the program's first variable, the FP32-prefilter level mask and children per
- * lane (prefilter_group), the dynamic cuboids per round (dequeue_buffer_size)
- * and the blocks per level launch (grid_blocks), starting from `base` (NULL: the defaults).  Candidates whose
- * counts differ from the plain binary64 run are rejected.  No counterpart in the
- * reference. */
+/* Chooses, by coordinate descent over timed proofs of a slice of the program's
+ * first variable, the FP32-prefilter level mask and children per lane
+ * (prefilter_group), the dynamic cuboids per round (dequeue_buffer_size) and the
+ * blocks per level launch (grid_blocks), starting from `base` (NULL: the
+ * defaults).  With base->jobs > 1 every timed proof is a batch of that many
+ * copies of the slice (sqd_prove_batch), as the proof service runs them, and the
+ * report's times are per batch.  Candidates whose counts differ from the plain
+ * binary64 run are rejected.  No counterpart in the reference. */
 typedef struct sqd_autotune_options {
     double target_seconds;    /* slice of the first variable to time: about this long, and about
                                  this much timing per candidate (default 0.02) */
@@ -302,6 +304,13 @@ typedef struct sqd_job_result {
 void sqd_service_options_default(sqd_service_options* options);
 int sqd_service_create(const sqd_service_options* options, sqd_service** out);
 int sqd_service_submit(sqd_service* service, const sqd_program* program, uint32_t first, uint32_t last, uint64_t* job);
+/* The settings the service's cores of `program` use (*known = 0 before its first
+ * job has chosen them), with jobs = batch_jobs. */
+int sqd_service_get_settings(sqd_service* service, const sqd_program* program, int* known, sqd_settings* out);
+/* Fixes the settings of `program`'s cores (instead of the defaults or autotuning);
+ * only before its first job.  level_bytes and frontier_capacity stay the
+ * service's, jobs is batch_jobs. */
+int sqd_service_set_settings(sqd_service* service, const sqd_program* program, const sqd_settings* settings);
 /* *done = 1 once the job has finished (its result is then copied to *out). */
 int sqd_service_poll(sqd_service* service, uint64_t job, int* done, sqd_job_result* out);
 /* Blocks until the job finishes, copies its result and forgets the job; returns its status. */

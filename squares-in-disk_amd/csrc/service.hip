@@ -9,8 +9,9 @@
 // memory budget; an idle core of another program is destroyed to make room.  A
 // job is admitted when a worker and a core (or the memory for one) are free.
 //
-// Autotuner: times sqd_prove_range on a slice of the program's first variable
-// and picks, by coordinate descent, the FP32-prefilter level mask, its children
+// Autotuner: times proofs of a slice of the program's first variable (batches
+// of sqd_settings.jobs copies of it on a batch core) and picks, by coordinate
+// descent, the FP32-prefilter level mask, its children
 // per lane, the dynamic cuboids per round (dequeue_buffer_size) and the blocks
 // per level launch.
 // These settings change the schedule, not the counts (counts do not depend on
@@ -67,7 +68,9 @@ bool same_counts(const sqd_counts& a, const sqd_counts& b) {
 
 // Median time of the proofs of [lo, hi) on a fresh core with settings s: two
 // untimed proofs first (the second captures the graphs), then at least `reps`
-// and as many as fit in about `budget` seconds (at most 200).
+// and as many as fit in about `budget` seconds (at most 200).  With s.jobs > 1
+// each proof is a batch of s.jobs copies of the range (the way the service runs
+// them), so that the kernels' share of the time is the batch's.
 Trial time_settings(int device, const sqd_program* prog, const sqd_settings& s, uint32_t lo, uint32_t hi, int reps,
                     double budget) {
     Trial t;
@@ -75,13 +78,18 @@ Trial time_settings(int device, const sqd_program* prog, const sqd_settings& s, 
     sqd_core* core = nullptr;
     t.status = sqd_core_create(device, prog, &s, &core);
     if(t.status != SQD_OK) return t;
+    const size_t jobs = s.jobs > 1 ? s.jobs : 1;
+    std::vector<uint32_t> first(jobs, lo), last(jobs, hi);
+    std::vector<sqd_job_counts> jc(jobs);
     std::vector<double> times;
     int want = reps;
     for(int r = 0; r < want + 2 && t.status == SQD_OK; ++r) {
-        size_t nc = 0;
         const auto t0 = Clock::now();
-        t.status = sqd_prove_range(core, lo, hi, &t.counts, nullptr, nullptr, 0, &nc, nullptr);
+        t.status = sqd_prove_batch(core, jobs, first.data(), last.data(), jc.data(), nullptr);
         const double dt = seconds_since(t0, Clock::now());
+        t.counts = jc[0].counts;
+        for(size_t j = 1; j < jobs && t.status == SQD_OK; ++j)
+            if(!same_counts(jc[j].counts, jc[0].counts)) t.status = SQD_ERR_INTERNAL;
         if(r == 1 && dt > 0) want = std::max(reps, std::min(200, static_cast<int>(budget / dt)));
         if(r >= 2) times.push_back(dt);
     }
@@ -117,9 +125,11 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         else sqd_settings_default(&s0);
         s0.fp32_prefilter = 0;
         // the tuning workload: the whole range, or a centred slice of it whose
-        // proof takes about target_seconds
+        // proof takes about target_seconds (sized on a core of one job)
+        sqd_settings s_probe = s0;
+        s_probe.jobs = 1;
         sqd_core* probe = nullptr;
-        int rc = sqd_core_create(device, prog, &s0, &probe);
+        int rc = sqd_core_create(device, prog, &s_probe, &probe);
         if(rc != SQD_OK) return rc;
         uint32_t all = 0, nstat = 0;
         rc = sqd_core_shape(probe, &all, &nstat);
@@ -131,6 +141,11 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         };
         uint32_t lo = 0, hi = all;
         centre(w, lo, hi);
+        if(rc == SQD_OK) {   // one untimed proof first: the first launches of a process load code
+            sqd_counts c{};
+            size_t nc = 0;
+            rc = sqd_prove_range(probe, lo, hi, &c, nullptr, nullptr, 0, &nc, nullptr);
+        }
         if(rc == SQD_OK) {
             for(int step = 0; step < 12; ++step) {
                 sqd_counts c{};
@@ -319,6 +334,7 @@ struct sqd_service {
                     tuning[prog] = true;
                     lk.unlock();
                     sqd_settings best = base_settings(), base = base_settings();
+                    base.jobs = opt.batch_jobs;       // tuned on batches as the workers run them
                     if(sqd_autotune(device, prog, &base, nullptr, &best, nullptr) != SQD_OK) best = base_settings();
                     best.level_bytes = opt.level_bytes;
                     best.frontier_capacity = opt.frontier_capacity;
@@ -435,6 +451,32 @@ int sqd_service_create(const sqd_service_options* opt, sqd_service** out) {
     } catch(const std::exception& e) {
         return fail(SQD_ERR_INTERNAL, e.what());
     }
+}
+
+int sqd_service_get_settings(sqd_service* s, const sqd_program* prog, int* known, sqd_settings* out) {
+    if(!s || !prog || !known || !out) return fail(SQD_ERR_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->settings.find(prog);
+    *known = it != s->settings.end() ? 1 : 0;
+    if(*known) {
+        *out = it->second;
+        out->jobs = s->opt.batch_jobs;
+    }
+    return SQD_OK;
+}
+
+int sqd_service_set_settings(sqd_service* s, const sqd_program* prog, const sqd_settings* settings) {
+    if(!s || !prog || !settings) return fail(SQD_ERR_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    if(s->tuning[prog]) return fail(SQD_ERR_ARGUMENT, "the program's settings are being tuned");
+    for(const auto& sl : s->pool)
+        if(sl.prog == prog) return fail(SQD_ERR_ARGUMENT, "the program already has cores; set its settings before its first job");
+    sqd_settings v = *settings;
+    v.level_bytes = s->opt.level_bytes;
+    v.frontier_capacity = s->opt.frontier_capacity;
+    s->settings[prog] = v;
+    s->cv.notify_all();
+    return SQD_OK;
 }
 
 int sqd_service_submit(sqd_service* s, const sqd_program* prog, uint32_t first, uint32_t last, uint64_t* job) {

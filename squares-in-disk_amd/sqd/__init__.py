@@ -148,6 +148,8 @@ def lib():
             "sqd_service_create": (ctypes.c_int, [ctypes.POINTER(ServiceOptions), ctypes.POINTER(_P)]),
             "sqd_service_submit": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_service_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(JobResult)]),
+            "sqd_service_get_settings": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Settings)]),
+            "sqd_service_set_settings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Settings)]),
             "sqd_service_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
             "sqd_service_destroy": (None, [_P]),
             "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -178,6 +180,7 @@ def exported_symbols():
             "sqd_prove_batch", "sqd_core_job_criticals",
             "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
             "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
+            "sqd_service_get_settings", "sqd_service_set_settings",
             "sqd_service_wait", "sqd_service_destroy",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
@@ -508,6 +511,17 @@ class Service:
             setattr(o, k, v)
         self._h = _P()
         _check(lib().sqd_service_create(ctypes.byref(o), ctypes.byref(self._h)))
+
+    def settings(self, program):
+        """The settings of the program's cores (dict), or None before its first job."""
+        known = ctypes.c_int()
+        s = Settings()
+        _check(lib().sqd_service_get_settings(self._h, program._h, ctypes.byref(known), ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Settings._fields_} if known.value else None
+
+    def set_settings(self, program, settings):
+        """Fixes the program's settings (a Settings) before its first job."""
+        _check(lib().sqd_service_set_settings(self._h, program._h, ctypes.byref(settings)))
 
     def submit(self, program, first, last):
         j = ctypes.c_uint64()

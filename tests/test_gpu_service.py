@@ -141,3 +141,23 @@ def test_progress_reports():
     for s in seen[:-1]:
         lb, ub = s["head"]
         assert s["queue_size"] > 0 and all(a <= b for a, b in zip(lb, ub))
+
+
+def test_service_fixed_settings_and_batches():
+    # settings fixed before the first job are the ones the cores use; batches of
+    # one program's queued jobs give each job its range's counts
+    p = sqd.Program.from_lemma("two")
+    s = sqd.default_settings(fp32_prefilter=2, prefilter_group=4, dequeue_buffer_size=4096)
+    ranges = [(0, 128), (0, 64), (64, 128), (5, 6), (0, 128), (100, 128), (1, 127)]
+    with sqd.Core(p) as core:
+        want = [core.prove_range(a, b)["counts"] for a, b in ranges]
+    with sqd.Service(workers=2, batch_jobs=3) as svc:
+        assert svc.settings(p) is None
+        svc.set_settings(p, s)
+        got = svc.settings(p)
+        assert (got["fp32_prefilter"], got["prefilter_group"], got["dequeue_buffer_size"], got["jobs"]) == (2, 4, 4096, 3)
+        ids = [svc.submit(p, a, b) for a, b in ranges]
+        res = [svc.wait(i) for i in ids]
+        with pytest.raises(sqd.SqdError):
+            svc.set_settings(p, s)                           # the program has cores now
+    assert [r["counts"] for r in res] == want
