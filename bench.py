@@ -75,7 +75,10 @@ def parse():
     ap.add_argument("--lemma", default="one", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-runs", type=int, default=3, help="timed oracle runs for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=None,
+    ap.add_argument("--batched-settings", default=None, metavar="K=V,...",
+                    help="fixed settings of the batched leg (e.g. fp32_prefilter=2,prefilter_group=8) instead of "
+                         "autotuning them; for reproducible profiles")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_static_level_one_subcontainer.txt"),
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight in the batched legs")
     ap.add_argument("--batch-jobs", type=int, default=8, help="proofs per batch (one grid row each)")
@@ -183,14 +186,21 @@ def initial_subdivisions(program):
     return 1
 
 
-def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_value, counted32):
+def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_value, counted32, fixed=None):
     """The batched FP32-first leg on `prog` (full proofs through the service)."""
     import ctypes
     svc_opts = sqd.ServiceOptions()
     sqd.lib().sqd_service_options_default(ctypes.byref(svc_opts))
     base = sqd.default_settings(jobs=batch_jobs, level_bytes=svc_opts.level_bytes,
                                 frontier_capacity=svc_opts.frontier_capacity)
-    best, tune = sqd.autotune(prog, base=base)            # as the service tunes it (reported)
+    if fixed:
+        best = base
+        for kv in fixed.split(","):
+            k, v = kv.split("=")
+            setattr(best, k.strip(), int(v, 0))
+        tune = {"settings": {f: getattr(best, f) for f, _ in sqd.Settings._fields_}, "first": None, "last": None}
+    else:
+        best, tune = sqd.autotune(prog, base=base)        # as the service tunes it (reported)
     tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "prefilter_group", "dequeue_buffer_size", "grid_blocks")}
     n = initial_subdivisions(prog)
     # decided fraction and the dominant prefiltered kernel's time on one core
@@ -225,7 +235,7 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
         "workers": workers, "batch_jobs": batch_jobs, "proofs": proofs, "seconds": elapsed,
         "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
         "run_ms": {"p50": percentile([r["run_seconds"] for r in res], 0.5) * 1e3},
-        "autotuned": tuned, "autotune_slice": [tune["first"], tune["last"]],
+        "autotuned" if not fixed else "settings": tuned, "autotune_slice": [tune["first"], tune["last"]],
         "fp32_checked": checked, "fp32_decided": decided,
         "fp32_decided_fraction": decided / checked if checked else None,
         "single_core_kernel_ms_per_proof": {s["name"]: s["milliseconds"] / 20 for s in
@@ -355,7 +365,7 @@ def main():
         counted32 = fp32_op_counts(prog.text())
     value = total_cuboids / elapsed
     batched = None if a.no_batched else fp32_first_batched(sqd, prog, expected, a.workers, a.batch_jobs,
-                                                           a.batched_proofs, value, counted32)
+                                                           a.batched_proofs, value, counted32, a.batched_settings)
     if batched is not None:
         batched["width_ratio_per_constraint"] = width_sample(sqd, prog)
     stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.batch_jobs, a.seed)
