@@ -68,9 +68,9 @@ typedef struct sqd_settings {
                                       everything in binary64) */
     uint32_t graphs;               /* nonzero: sqd_prove replays its rounds as captured HIP
                                       graphs from the second proof on a core (default 1) */
-    uint32_t prefilter_group;      /* FP32 prefilter: most children of one parent a lane
-                                      evaluates together, sharing the terms free of the
-                                      split variable (1, 2, 4, 8; 0 = chosen per launch) */
+    uint32_t prefilter_group;      /* FP32 prefilter: consecutive children of one parent a
+                                      lane first tests as one group (a power of two up to
+                                      64; 0 = chosen per launch, 1 = no groups) */
     uint32_t jobs;                 /* jobs per batch (sqd_prove_batch), at most 64; the core
                                       holds that many copies of its device memory and runs
                                       every launch with one grid row per job; 0 or 1: one */

@@ -368,10 +368,12 @@ __device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, u
 }
 
 // ---------------------------------------------------------------- static levels
-// Most children of one parent that one lane of the FP32 prefilter handles (the
-// terms free of the level's variable are evaluated once for them).
+// Largest group of consecutive children of one parent that one lane of the
+// FP32 prefilter tests at once (a power of two dividing SUBS, at most 64).
 template<int SUBS> constexpr int children_per_lane() {
-    return SUBS % 8 == 0 ? 8 : (SUBS % 4 == 0 ? 4 : (SUBS % 2 == 0 ? 2 : 1));
+    int c = 64;
+    while(c > 1 && SUBS % c != 0) c >>= 1;
+    return c;
 }
 
 // Child w of a static level: the parent box with the level's variable replaced
@@ -597,9 +599,19 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
 // which decides them in binary64; survivors, hulls and criticals therefore
 // always come from the binary64 evaluation.
 //
-// On the hybrid box every term free of A is the same for all children of a
-// parent, so each lane takes up to CPL children of one parent and evaluates those
-// terms once (P::hoist), then the rest per child (P::rows_violated_hoisted).
+// Groups: each lane first takes a group of cpl consecutive children of one
+// parent (cpl from args.children_per_lane or the launch size, at most CPL) and
+// checks the rows once on the parent's box with A over the whole group, the
+// terms free of A evaluated once (P::hoist).  The outward-rounded operations are
+// inclusion-isotone (each bound is the directed rounding of the exact bound, a
+// monotone function of the operands' bounds), and every child's hybrid box lies
+// in the group's box, so a group that fails a row holds only children that the
+// per-child test prunes too: they are counted as pruned by it.  The children of
+// the other groups are gathered in LDS and spread over the block's lanes for
+// the per-child test (propagation, then the rows on the hybrid box); the
+// results, and the counts of checked and decided children, are those of the
+// per-child test alone.  Propagation is not part of the group test: its
+// iteration depends on which bounds changed, so it is not isotone.
 // The binary32 pass runs in its own kernel so that its register budget (and
 // with it the occupancy) is not that of the binary64 code.
 template<class P, int MODE, int L>
@@ -618,60 +630,90 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-    // children per lane: the most that still leave two lane slots per thread of
-    // the grid (small launches need the parallelism more than the hoisting)
+    // children per group: as asked, or the most up to 16 that still leave two
+    // groups per thread of the grid (small launches need the parallelism more)
     int cpl = 1;
     for(int c = CPL; c > 1; c >>= 1) {
         if(SUBS % c != 0) continue;
         if(args.children_per_lane > 0 ? c <= args.children_per_lane
-                                      : n_items * static_cast<unsigned long long>(SUBS / c) >= 2 * stride) {
+                                      : c <= 16 && n_items * static_cast<unsigned long long>(SUBS / c) >= 2 * stride) {
             cpl = c;
             break;
         }
     }
-    const int step = SUBS / cpl;                                                       // lanes per parent
-    const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // lane slots
+    const int step = SUBS / cpl;                                                       // groups per parent
+    const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const unsigned long long trips = (total + stride - 1) / stride;
+    __shared__ unsigned long long s_groups[kBlock];   // the trip's groups that the group test left
+    __shared__ unsigned long long s_ngroups;
     unsigned long long c_cub = 0, c_leaf = 0, c_dec = 0, c_chk = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
+        if(threadIdx.x == 0) s_ngroups = 0;
+        __syncthreads();
+        // 1. group test: the lane's group is children [g * cpl, (g + 1) * cpl) of
+        //    one parent; the rows are checked once on the parent's box with the
+        //    level's variable over the whole group
         const unsigned long long slot = first + trip * stride;
-        const bool valid = slot < total;
-        const unsigned long long item = slot / static_cast<unsigned long long>(step);
-        const int j = static_cast<int>(slot % static_cast<unsigned long long>(step));
-        float plb[N], pub[N];
-        double pl = 0, pu = 0, wd = 0;
-        typename P::template Hoist<L, float> h;
-        if(valid) {
-            double lb[N], ub[N];
-            load_box<P>(args.in, in_base + item, lb, ub);
+        bool keep = false;
+        if(slot < total) {
+            const unsigned long long item = slot / static_cast<unsigned long long>(step);
+            const int g = static_cast<int>(slot % static_cast<unsigned long long>(step));
+            if(g == 0) c_cub += SUBS;
+            keep = true;
+            if(cpl > 1) {
+                double lb[N], ub[N];
+                load_box<P>(args.in, in_base + item, lb, ub);
+                float hl[N], hu[N];
 #pragma unroll
-            for(int a = 0; a < N; ++a) { plb[a] = f32_down(lb[a]); pub[a] = f32_up(ub[a]); }
-            pl = lb[ARG];
-            pu = ub[ARG];
-            wd = split_width(pl, pu, SUBS);
-            P::template hoist<L>(plb, pub, s_c32, h);
+                for(int a = 0; a < N; ++a) { hl[a] = f32_down(lb[a]); hu[a] = f32_up(ub[a]); }
+                typename P::template Hoist<L, float> h;
+                P::template hoist<L>(hl, hu, s_c32, h);
+                const double wd = split_width(lb[ARG], ub[ARG], SUBS);
+                hl[ARG] = f32_down(split_point(lb[ARG], ub[ARG], wd, g * cpl, SUBS));
+                hu[ARG] = f32_up(split_point(lb[ARG], ub[ARG], wd, (g + 1) * cpl, SUBS));
+                if(P::template rows_violated_hoisted<L>(hl, hu, s_c32, h)) {
+                    keep = false;
+                    c_chk += static_cast<unsigned long long>(cpl);
+                    c_leaf += static_cast<unsigned long long>(cpl);
+                    c_dec += static_cast<unsigned long long>(cpl);
+                }
+            }
         }
-#pragma unroll 1
-        for(int k = 0; k < cpl; ++k) {
-            const int i = j + k * step;
+        const unsigned long long at = wave_append(keep, &s_ngroups);
+        if(keep) s_groups[at] = slot;
+        __syncthreads();
+        // 2. the children of the groups left, spread over the block's lanes
+        const unsigned long long m = s_ngroups * static_cast<unsigned long long>(cpl);
+        for(unsigned long long base = 0; base < m; base += blockDim.x) {
+            const unsigned long long q = base + threadIdx.x;
             bool undecided = false;
-            if(valid) {
-                if(i == 0) c_cub += SUBS;
+            unsigned long long w = 0;
+            if(q < m) {
+                const unsigned long long gs = s_groups[q / static_cast<unsigned long long>(cpl)];
+                const unsigned long long item = gs / static_cast<unsigned long long>(step);
+                const int i = static_cast<int>(gs % static_cast<unsigned long long>(step)) * cpl +
+                              static_cast<int>(q % static_cast<unsigned long long>(cpl));
+                w = item * SUBS + static_cast<unsigned long long>(i);
+                double lb[N], ub[N];
+                load_box<P>(args.in, in_base + item, lb, ub);
+                float plb[N], pub[N];
+#pragma unroll
+                for(int a = 0; a < N; ++a) { plb[a] = f32_down(lb[a]); pub[a] = f32_up(ub[a]); }
+                const double wd = split_width(lb[ARG], ub[ARG], SUBS);
                 float lb32[N], ub32[N];
 #pragma unroll
                 for(int a = 0; a < N; ++a) { lb32[a] = plb[a]; ub32[a] = pub[a]; }
-                lb32[ARG] = f32_down(split_point(pl, pu, wd, i, SUBS));
-                ub32[ARG] = f32_up(split_point(pl, pu, wd, i + 1, SUBS));
+                lb32[ARG] = f32_down(split_point(lb[ARG], ub[ARG], wd, i, SUBS));
+                ub32[ARG] = f32_up(split_point(lb[ARG], ub[ARG], wd, i + 1, SUBS));
                 ++c_chk;
                 bool pruned = propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop);
                 if(!pruned) {
-                    float hl[N], hu[N];
-#pragma unroll
-                    for(int a = 0; a < N; ++a) { hl[a] = plb[a]; hu[a] = pub[a]; }
-                    hl[ARG] = lb32[ARG];
-                    hu[ARG] = ub32[ARG];
-                    pruned = P::template rows_violated_hoisted<L>(hl, hu, s_c32, h);
+                    typename P::template Hoist<L, float> h;
+                    P::template hoist<L>(plb, pub, s_c32, h);
+                    plb[ARG] = lb32[ARG];
+                    pub[ARG] = ub32[ARG];
+                    pruned = P::template rows_violated_hoisted<L>(plb, pub, s_c32, h);
                 }
                 if(pruned) {
                     ++c_leaf;
@@ -680,13 +722,13 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
                     undecided = true;
                 }
             }
-            const unsigned long long w = item * SUBS + static_cast<unsigned long long>(i);
             const unsigned long long pos = wave_append(undecided, args.undec_count);
             if(undecided) {
                 if(pos < args.undec_cap) args.undec[pos] = w;
                 else atomicAdd(&args.counters->overflow, 1ull);
             }
         }
+        __syncthreads();
     }
     block_counts_add(args.block_counts, c_cub, c_leaf, 0ull, c_dec, c_chk);
 }

@@ -128,6 +128,7 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         // proof takes about target_seconds (sized on a core of one job)
         sqd_settings s_probe = s0;
         s_probe.jobs = 1;
+        s_probe.graphs = 0;   // a graph capture would make one of the timed proofs slow
         sqd_core* probe = nullptr;
         int rc = sqd_core_create(device, prog, &s_probe, &probe);
         if(rc != SQD_OK) return rc;
@@ -148,11 +149,15 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         }
         if(rc == SQD_OK) {
             for(int step = 0; step < 12; ++step) {
-                sqd_counts c{};
-                size_t nc = 0;
-                const auto t0 = Clock::now();
-                rc = sqd_prove_range(probe, lo, hi, &c, nullptr, nullptr, 0, &nc, nullptr);
-                const double dt = seconds_since(t0, Clock::now());
+                double dt = 0;
+                for(int r = 0; r < 2 && rc == SQD_OK; ++r) {   // the faster of two proofs
+                    sqd_counts c{};
+                    size_t nc = 0;
+                    const auto t0 = Clock::now();
+                    rc = sqd_prove_range(probe, lo, hi, &c, nullptr, nullptr, 0, &nc, nullptr);
+                    const double d = seconds_since(t0, Clock::now());
+                    dt = r == 0 ? d : std::min(dt, d);
+                }
                 if(rc != SQD_OK) break;
                 if(dt > o.target_seconds && w > 1) w = std::max<uint32_t>(1, w / 2);
                 else if(dt < o.target_seconds / 4 && w < all) w = std::min(all, w * 2);
@@ -189,7 +194,7 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         // 2. children per lane of the prefilter (when a level uses it)
         if(cur.s.fp32_prefilter) {
             const sqd_settings s_p = cur.s;
-            for(uint32_t g : {1u, 2u, 4u, 8u}) {
+            for(uint32_t g : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
                 sqd_settings s = s_p;
                 s.prefilter_group = g;
                 consider(s);
