@@ -74,7 +74,12 @@ typedef struct sqd_settings {
     uint32_t jobs;                 /* jobs per batch (sqd_prove_batch), at most 64; the core
                                       holds that many copies of its device memory and runs
                                       every launch with one grid row per job; 0 or 1: one */
-    uint32_t reserved;
+    uint32_t group_levels;         /* bit mask of the static levels (bit L: level L) decided
+                                      in binary64 with the group test: the rows are first
+                                      checked once per group of prefilter_group consecutive
+                                      children, the children of the groups left one by one;
+                                      exactly the results of the plain pass (no counterpart in
+                                      the reference; levels in fp32_prefilter take that) */
 } sqd_settings;
 
 #define SQD_FP32_ALL_LEVELS 0xffffffffu
@@ -241,8 +246,9 @@ int sqd_core_shape(const sqd_core* core, uint32_t* initial_subdivisions, uint32_
 /* ---------------------------------------------------------------- autotuner */
 
 /* Chooses, by coordinate descent over timed proofs of a slice of the program's
- * first variable, the FP32-prefilter level mask and children per lane
- * (prefilter_group), the dynamic cuboids per round (dequeue_buffer_size) and the
+ * first variable, the FP32-prefilter level mask, the binary64 group-test level
+ * mask (group_levels), the children per group (prefilter_group), the dynamic
+ * cuboids per round (dequeue_buffer_size) and the
  * blocks per level launch (grid_blocks), starting from `base` (NULL: the
  * defaults).  With base->jobs > 1 every timed proof is a batch of that many
  * copies of the slice (sqd_prove_batch), as the proof service runs them, and the

@@ -51,6 +51,8 @@ enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
 struct NoHoist {};
 // Flag or'ed into a static-level mode at launch: the FP32-prefiltered kernel.
 constexpr int kModeF32 = 4;
+// Static-level mode bit: binary64 with the group test (k_static_grouped).
+constexpr int kModeGroup = 8;
 
 struct DeviceCounters {
     unsigned long long cuboids;
@@ -368,12 +370,25 @@ __device__ __forceinline__ void hull_join(unsigned long long* hull, bool join, u
 }
 
 // ---------------------------------------------------------------- static levels
-// Largest group of consecutive children of one parent that one lane of the
-// FP32 prefilter tests at once (a power of two dividing SUBS, at most 64).
+// Largest group of consecutive children of one parent that one lane tests at
+// once (a power of two dividing SUBS, at most 64).
 template<int SUBS> constexpr int children_per_lane() {
     int c = 64;
     while(c > 1 && SUBS % c != 0) c >>= 1;
     return c;
+}
+// Children per group of a launch: as args.children_per_lane asks, or the most up
+// to 16 that still leave two groups per thread of the grid (small launches need
+// the parallelism more).
+template<int SUBS, int CPL>
+__device__ __forceinline__ int group_size(const LevelArgs& args, unsigned long long n_items, unsigned long long stride) {
+    for(int c = CPL; c > 1; c >>= 1) {
+        if(SUBS % c != 0) continue;
+        if(args.children_per_lane > 0 ? c <= args.children_per_lane
+                                      : c <= 16 && n_items * static_cast<unsigned long long>(SUBS / c) >= 2 * stride)
+            return c;
+    }
+    return 1;
 }
 
 // Child w of a static level: the parent box with the level's variable replaced
@@ -630,17 +645,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-    // children per group: as asked, or the most up to 16 that still leave two
-    // groups per thread of the grid (small launches need the parallelism more)
-    int cpl = 1;
-    for(int c = CPL; c > 1; c >>= 1) {
-        if(SUBS % c != 0) continue;
-        if(args.children_per_lane > 0 ? c <= args.children_per_lane
-                                      : c <= 16 && n_items * static_cast<unsigned long long>(SUBS / c) >= 2 * stride) {
-            cpl = c;
-            break;
-        }
-    }
+    const int cpl = group_size<SUBS, CPL>(args, n_items, stride);
     const int step = SUBS / cpl;                                                       // groups per parent
     const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -731,6 +736,84 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
         __syncthreads();
     }
     block_counts_add(args.block_counts, c_cub, c_leaf, 0ull, c_dec, c_chk);
+}
+
+// Level L of the static grid in binary64 with the group test (the levels of
+// sqd_settings.group_levels that the FP32 prefilter does not take): each lane
+// first checks the rows once, in binary64, on the parent's box with A over a
+// group of cpl consecutive children (the terms free of A hoisted); the children
+// of the groups left are decided one by one exactly as k_static_level decides
+// them (static_decide), spread over the block's lanes.  The binary64 operations
+// are inclusion-isotone and every child's propagated box lies in the group's
+// box, so a group that fails a row holds only children that static_decide
+// prunes too (by propagation or by a row): survivors, hulls, criticals and
+// counts are exactly those of k_static_level.
+template<class P, int MODE, int L>
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_grouped(LevelArgs job0) {
+    const LevelArgs args = job_args(job0);
+    constexpr int N = P::N;
+    constexpr int ARG = P::st_arg(L);
+    constexpr int SUBS = P::st_subs(L);
+    constexpr int CPL = children_per_lane<SUBS>();
+    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    __shared__ PropScratch<P> s_prop;
+    __shared__ unsigned long long s_groups[kBlock];   // the trip's groups that the group test left
+    __shared__ unsigned long long s_ngroups;
+    load_consts<P>(s_c, args.consts);
+    enter_round_down();
+    const unsigned long long n_items = *args.in_count;
+    const unsigned long long in_base = input_base(args);
+    const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+    const int cpl = group_size<SUBS, CPL>(args, n_items, stride);
+    const int step = SUBS / cpl;                                                       // groups per parent
+    const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
+    const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned long long trips = (total + stride - 1) / stride;
+    unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
+    for(unsigned long long trip = 0; trip < trips; ++trip) {
+        if(threadIdx.x == 0) s_ngroups = 0;
+        __syncthreads();
+        const unsigned long long slot = first + trip * stride;
+        bool keep = false;
+        if(slot < total) {
+            const unsigned long long item = slot / static_cast<unsigned long long>(step);
+            const int g = static_cast<int>(slot % static_cast<unsigned long long>(step));
+            if(g == 0) c_cub += SUBS;
+            keep = true;
+            if(cpl > 1) {
+                double lb[N], ub[N];
+                load_box<P>(args.in, in_base + item, lb, ub);
+                typename P::template Hoist<L, double> h;
+                P::template hoist<L>(lb, ub, s_c, h);
+                const double pl = lb[ARG], pu = ub[ARG];
+                const double wd = split_width(pl, pu, SUBS);
+                lb[ARG] = split_point(pl, pu, wd, g * cpl, SUBS);
+                ub[ARG] = split_point(pl, pu, wd, (g + 1) * cpl, SUBS);
+                if(P::template rows_violated_hoisted<L>(lb, ub, s_c, h)) {
+                    keep = false;
+                    c_leaf += static_cast<unsigned long long>(cpl);
+                }
+            }
+        }
+        const unsigned long long at = wave_append(keep, &s_ngroups);
+        if(keep) s_groups[at] = slot;
+        __syncthreads();
+        const unsigned long long m = s_ngroups * static_cast<unsigned long long>(cpl);
+        for(unsigned long long base = 0; base < m; base += blockDim.x) {   // block-uniform trip count
+            const unsigned long long q = base + threadIdx.x;
+            const bool valid = q < m;
+            unsigned long long w = 0;
+            if(valid) {
+                const unsigned long long gs = s_groups[q / static_cast<unsigned long long>(cpl)];
+                w = (gs / static_cast<unsigned long long>(step)) * SUBS +
+                    (gs % static_cast<unsigned long long>(step)) * static_cast<unsigned long long>(cpl) +
+                    q % static_cast<unsigned long long>(cpl);
+            }
+            static_decide<P, MODE, L>(args, in_base, valid, w, s_c, s_prop, c_leaf, c_crit);
+        }
+        __syncthreads();
+    }
+    block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
 }
 
 // ---------------------------------------------------------------- dynamic levels

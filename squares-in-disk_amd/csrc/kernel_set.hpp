@@ -19,7 +19,8 @@ struct KernelSet {
     // launch (host-known upper bound); `cus` is the device's compute-unit count.
     // The grid is the kernel's resident capacity (occupancy x CUs), capped by the
     // blocks the bound needs; a nonzero `grid` overrides it.  A static-level mode
-    // or'ed with dev::kModeF32 selects the FP32-prefiltered variant.
+    // or'ed with dev::kModeF32 selects the FP32-prefiltered variant, with
+    // dev::kModeGroup the binary64 group test.
     hipError_t (*static_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
                                unsigned cus, unsigned grid, hipStream_t s);
     hipError_t (*dynamic_level)(int mode, int level, const dev::LevelArgs& a, unsigned long long max_children,
@@ -86,6 +87,22 @@ hipError_t launch_one_static(const dev::LevelArgs& a, unsigned long long max_chi
                        dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
+template<class P, int MODE, int L>
+hipError_t launch_one_grouped(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
+                              hipStream_t s) {
+    static const unsigned per_cu = blocks_per_cu(dev::k_static_grouped<P, MODE, L>);
+    hipLaunchKernelGGL((dev::k_static_grouped<P, MODE, L>),
+                       dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
+    return hipGetLastError();
+}
+template<class P, int MODE, int... L>
+hipError_t launch_grouped(int level, const dev::LevelArgs& a, unsigned long long mc, unsigned cus, unsigned grid,
+                          hipStream_t s, std::integer_sequence<int, L...>) {
+    hipError_t e = hipErrorInvalidValue;
+    (void)a; (void)mc; (void)cus; (void)grid; (void)s; (void)level;
+    ((level == L ? (e = launch_one_grouped<P, MODE, L>(a, mc, cus, grid, s), 0) : 0), ...);
+    return e;
+}
 template<class P, int MODE, int J>
 hipError_t launch_one_dynamic(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
                               hipStream_t s) {
@@ -120,6 +137,8 @@ hipError_t static_level(int mode, int level, const dev::LevelArgs& a, unsigned l
     if(mode == (dev::NONFINAL | dev::kModeF32))
         return launch_static<P, dev::NONFINAL, true>(level, a, mc, cus, grid, s, seq);
     if(mode == (dev::FINAL | dev::kModeF32)) return launch_static<P, dev::FINAL, true>(level, a, mc, cus, grid, s, seq);
+    if(mode == (dev::NONFINAL | dev::kModeGroup)) return launch_grouped<P, dev::NONFINAL>(level, a, mc, cus, grid, s, seq);
+    if(mode == (dev::FINAL | dev::kModeGroup)) return launch_grouped<P, dev::FINAL>(level, a, mc, cus, grid, s, seq);
     return hipErrorInvalidValue;
 }
 template<class P>

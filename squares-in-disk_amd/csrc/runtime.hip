@@ -488,7 +488,9 @@ void run_static(sqd_core* c, int mode, const dev::BoxList& in, const unsigned lo
         const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
         launch(c, mode * 16 + L, "static level launch",
                [&] {
-                   const int m = mode | (((c->settings.fp32_prefilter >> L) & 1u) ? dev::kModeF32 : 0);
+                   const bool f32 = (c->settings.fp32_prefilter >> L) & 1u;
+                   const bool grp = (c->settings.group_levels >> L) & 1u;
+                   const int m = mode | (f32 ? dev::kModeF32 : grp ? dev::kModeGroup : 0);
                    return c->ks->static_level(m, L, a, children, c->cus, c->grid_override, c->stream);
                });
         prev = out;

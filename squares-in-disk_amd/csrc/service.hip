@@ -11,9 +11,9 @@
 //
 // Autotuner: times proofs of a slice of the program's first variable (batches
 // of sqd_settings.jobs copies of it on a batch core) and picks, by coordinate
-// descent, the FP32-prefilter level mask, its children
-// per lane, the dynamic cuboids per round (dequeue_buffer_size) and the blocks
-// per level launch.
+// descent, the FP32-prefilter level mask, the binary64 group-test level mask,
+// the children per group, the dynamic cuboids per round (dequeue_buffer_size)
+// and the blocks per level launch.
 // These settings change the schedule, not the counts (counts do not depend on
 // the order or the batching of the frontier); the prefilter could prune a child
 // binary64 keeps, so candidates whose counts differ from the plain run's are
@@ -124,6 +124,7 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
         if(base) s0 = *base;
         else sqd_settings_default(&s0);
         s0.fp32_prefilter = 0;
+        s0.group_levels = 0;
         // the tuning workload: the whole range, or a centred slice of it whose
         // proof takes about target_seconds (sized on a core of one job)
         sqd_settings s_probe = s0;
@@ -191,8 +192,16 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
             s.fp32_prefilter = m;
             consider(s);
         }
-        // 2. children per lane of the prefilter (when a level uses it)
-        if(cur.s.fp32_prefilter) {
+        // 2. binary64 group test on the levels the prefilter does not take
+        const sqd_settings s_grp = cur.s;
+        for(uint32_t m = 1; m < masks; ++m) {
+            if(m & s_grp.fp32_prefilter) continue;
+            sqd_settings s = s_grp;
+            s.group_levels = m;
+            consider(s);
+        }
+        // 3. children per group (when a level groups them)
+        if(cur.s.fp32_prefilter || cur.s.group_levels) {
             const sqd_settings s_p = cur.s;
             for(uint32_t g : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
                 sqd_settings s = s_p;
@@ -200,14 +209,14 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
                 consider(s);
             }
         }
-        // 3. dynamic cuboids per round
+        // 4. dynamic cuboids per round
         const sqd_settings s_k = cur.s;
         for(uint32_t k : {1024u, 4096u, 16384u, 65536u}) {
             sqd_settings s = s_k;
             s.dequeue_buffer_size = k;
             consider(s);
         }
-        // 4. blocks per level launch (0: the kernel's resident capacity)
+        // 5. blocks per level launch (0: the kernel's resident capacity)
         const sqd_settings s_g = cur.s;
         for(uint32_t g : {cus, 2 * cus, 4 * cus}) {
             if(g == 0) continue;

@@ -38,7 +38,7 @@ class Settings(ctypes.Structure):
                 ("frontier_capacity", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64),
                 ("critical_capacity", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint32),
                 ("fp32_prefilter", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("prefilter_group", ctypes.c_uint32),
-                ("jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("jobs", ctypes.c_uint32), ("group_levels", ctypes.c_uint32)]
 
 
 FP32_ALL_LEVELS = 0xFFFFFFFF
