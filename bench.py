@@ -36,7 +36,7 @@ checked against the published ones.  The line printed carries
               it: p50/p99 latency and cuboids/s; every job's counts are checked
               against the sum of its subtrees' (a sequential binary64 run)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one] [--workers 4] [--batch-jobs 8]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lemma one] [--workers 4] [--batch-jobs 16]
 """
 import argparse
 import json
@@ -81,7 +81,9 @@ def parse():
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_static_level_one_subcontainer.txt"),
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight in the batched legs")
-    ap.add_argument("--batch-jobs", type=int, default=8, help="proofs per batch (one grid row each)")
+    ap.add_argument("--batch-jobs", type=int, default=16, help="proofs per batch (one grid row each)")
+    ap.add_argument("--stream-batch-jobs", type=int, default=8,
+                    help="jobs per batch in the job-stream leg (11 programs share the memory budget)")
     ap.add_argument("--batched-proofs", type=int, default=1024)
     ap.add_argument("--stream-jobs", type=int, default=48)
     ap.add_argument("--seed", type=int, default=20261202)
@@ -368,7 +370,7 @@ def main():
                                                            a.batched_proofs, value, counted32, a.batched_settings)
     if batched is not None:
         batched["width_ratio_per_constraint"] = width_sample(sqd, prog)
-    stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.batch_jobs, a.seed)
+    stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.stream_batch_jobs, a.seed)
 
     dom = next(s for s in stats if s["mode"] == dom["mode"] and s["level"] == dom["level"])
     mean_ms = dom["milliseconds"] / dom["launches"]
