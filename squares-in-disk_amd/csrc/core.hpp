@@ -51,7 +51,7 @@ enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
 struct NoHoist {};
 // Flag or'ed into a static-level mode at launch: the FP32-prefiltered kernel.
 constexpr int kModeF32 = 4;
-// Static-level mode bit: binary64 with the group test (k_static_grouped).
+// Static-level mode bit: binary64 with the group test (k_static_group64).
 constexpr int kModeGroup = 8;
 
 struct DeviceCounters {
@@ -240,6 +240,17 @@ __device__ __forceinline__ unsigned long long wave_append(bool flag, unsigned lo
     if(lane == first && mask) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
     base = __shfl(base, first);
     return base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+// The same with n entries for every flagged lane (n equal on all lanes).
+__device__ __forceinline__ unsigned long long wave_append_n(bool flag, unsigned n, unsigned long long* counter) {
+    const unsigned long long mask = __ballot(flag);
+    const unsigned long long active = __ballot(1);
+    const int lane = __lane_id();
+    const int first = __ffsll(static_cast<long long>(active)) - 1;
+    unsigned long long base = 0;
+    if(lane == first && mask) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)) * n);
+    base = __shfl(base, first);
+    return base + static_cast<unsigned long long>(__popcll(mask & ((1ull << lane) - 1ull))) * n;
 }
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
@@ -529,8 +540,9 @@ constexpr bool wave_hoisting() {
 // Where wave_hoisting<P, L>() holds, each wave takes whole parents: it computes
 // the parent's split-free terms once into its LDS slot, then decides the
 // SUBS children in trips of 64 (static_decide_hoisted).
-// F32 = true: the children listed by k_static_prefilter (those binary32 left
-// undecided) are decided in binary64, exactly as above.
+// F32 = true: the children listed in args.undec (by k_static_prefilter, those
+// binary32 left undecided; by k_static_group64, those of the groups that passed
+// the group test) are decided in binary64, exactly as above.
 template<class P, int MODE, int L, bool F32>
 __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs job0) {
     const LevelArgs args = job_args(job0);
@@ -738,27 +750,26 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     block_counts_add(args.block_counts, c_cub, c_leaf, 0ull, c_dec, c_chk);
 }
 
-// Level L of the static grid in binary64 with the group test (the levels of
+// The binary64 group test of static level L (the levels of
 // sqd_settings.group_levels that the FP32 prefilter does not take): each lane
-// first checks the rows once, in binary64, on the parent's box with A over a
-// group of cpl consecutive children (the terms free of A hoisted); the children
-// of the groups left are decided one by one exactly as k_static_level decides
-// them (static_decide), spread over the block's lanes.  The binary64 operations
-// are inclusion-isotone and every child's propagated box lies in the group's
-// box, so a group that fails a row holds only children that static_decide
-// prunes too (by propagation or by a row): survivors, hulls, criticals and
-// counts are exactly those of k_static_level.
+// checks the rows once, in binary64, on the parent's box with A over a group of
+// cpl consecutive children (the terms free of A hoisted), and appends the
+// children of a group that passes to args.undec; k_static_level<..., true> then
+// decides them one by one exactly as the plain pass does.  The binary64
+// operations are inclusion-isotone and every child's propagated box lies in the
+// group's box, so a group that fails a row holds only children that the plain
+// pass prunes too (by propagation or by a row): they are counted as its pruned
+// leaves, and survivors, hulls, criticals and counts are exactly the plain
+// pass's.  The test runs in its own kernel so that neither pass carries the
+// other's registers.
 template<class P, int MODE, int L>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_grouped(LevelArgs job0) {
+__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_group64(LevelArgs job0) {
     const LevelArgs args = job_args(job0);
     constexpr int N = P::N;
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
     constexpr int CPL = children_per_lane<SUBS>();
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
-    __shared__ PropScratch<P> s_prop;
-    __shared__ unsigned long long s_groups[kBlock];   // the trip's groups that the group test left
-    __shared__ unsigned long long s_ngroups;
     load_consts<P>(s_c, args.consts);
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
@@ -769,16 +780,16 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_grouped(LevelA
     const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const unsigned long long trips = (total + stride - 1) / stride;
-    unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
+    unsigned long long c_cub = 0, c_leaf = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
-        if(threadIdx.x == 0) s_ngroups = 0;
-        __syncthreads();
         const unsigned long long slot = first + trip * stride;
         bool keep = false;
+        unsigned long long w0 = 0;
         if(slot < total) {
             const unsigned long long item = slot / static_cast<unsigned long long>(step);
             const int g = static_cast<int>(slot % static_cast<unsigned long long>(step));
             if(g == 0) c_cub += SUBS;
+            w0 = item * SUBS + static_cast<unsigned long long>(g * cpl);
             keep = true;
             if(cpl > 1) {
                 double lb[N], ub[N];
@@ -795,25 +806,16 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_grouped(LevelA
                 }
             }
         }
-        const unsigned long long at = wave_append(keep, &s_ngroups);
-        if(keep) s_groups[at] = slot;
-        __syncthreads();
-        const unsigned long long m = s_ngroups * static_cast<unsigned long long>(cpl);
-        for(unsigned long long base = 0; base < m; base += blockDim.x) {   // block-uniform trip count
-            const unsigned long long q = base + threadIdx.x;
-            const bool valid = q < m;
-            unsigned long long w = 0;
-            if(valid) {
-                const unsigned long long gs = s_groups[q / static_cast<unsigned long long>(cpl)];
-                w = (gs / static_cast<unsigned long long>(step)) * SUBS +
-                    (gs % static_cast<unsigned long long>(step)) * static_cast<unsigned long long>(cpl) +
-                    q % static_cast<unsigned long long>(cpl);
+        const unsigned long long pos = wave_append_n(keep, static_cast<unsigned>(cpl), args.undec_count);
+        if(keep) {
+            if(pos + static_cast<unsigned long long>(cpl) <= args.undec_cap) {
+                for(int k = 0; k < cpl; ++k) args.undec[pos + k] = w0 + static_cast<unsigned long long>(k);
+            } else {
+                atomicAdd(&args.counters->overflow, 1ull);
             }
-            static_decide<P, MODE, L>(args, in_base, valid, w, s_c, s_prop, c_leaf, c_crit);
         }
-        __syncthreads();
     }
-    block_counts_add(args.block_counts, c_cub, c_leaf, MODE == FINAL ? c_crit : 0ull);
+    block_counts_add(args.block_counts, c_cub, c_leaf, 0ull);
 }
 
 // ---------------------------------------------------------------- dynamic levels

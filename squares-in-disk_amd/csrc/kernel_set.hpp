@@ -87,11 +87,18 @@ hipError_t launch_one_static(const dev::LevelArgs& a, unsigned long long max_chi
                        dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }
+// One static level with the binary64 group test: the test, then the binary64
+// kernel over the children of the groups it left.
 template<class P, int MODE, int L>
 hipError_t launch_one_grouped(const dev::LevelArgs& a, unsigned long long max_children, unsigned cus, unsigned grid,
                               hipStream_t s) {
-    static const unsigned per_cu = blocks_per_cu(dev::k_static_grouped<P, MODE, L>);
-    hipLaunchKernelGGL((dev::k_static_grouped<P, MODE, L>),
+    static const unsigned pre_cu = blocks_per_cu(dev::k_static_group64<P, MODE, L>);
+    hipLaunchKernelGGL((dev::k_static_group64<P, MODE, L>),
+                       dim3(level_grid(pre_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if(e != hipSuccess) return e;
+    static const unsigned per_cu = blocks_per_cu(dev::k_static_level<P, MODE, L, true>);
+    hipLaunchKernelGGL((dev::k_static_level<P, MODE, L, true>),
                        dim3(level_grid(per_cu * cus, max_children, grid), jobs_of(a)), dim3(dev::kBlock), 0, s, a);
     return hipGetLastError();
 }

@@ -930,7 +930,7 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->scratch.lay(lay, c->N, std::max<unsigned long long>(kmax, 1024), true, false);
         lay.take(c->d_hull, static_cast<std::size_t>(kmax) * c->N * 4);
         lay.take(c->d_range, 2);
-        if(c->fp32) {   // as many undecided children as a level buffer holds boxes
+        if(c->fp32 || (s.group_levels & used)) {   // as many undecided children as a level buffer holds boxes
             c->undec_cap = lvl_cap;
             lay.take(c->d_undec, static_cast<std::size_t>(lvl_cap));
         }
