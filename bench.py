@@ -31,7 +31,8 @@ checked against the published ones.  The line printed carries
               FP32-versus-FP64 width ratio of each constraint on sampled boxes
   job_stream  proof jobs over all 11 statements (statement uniform, a
               contiguous 1/2^k slice of the first variable's initial
-              subdivisions, k uniform in 0..6, seeded), first run as one batch to
+              subdivisions, k uniform in 0..6, seeded) through a service with a
+              64 GiB device-memory budget, first run as one batch to
               measure capacity, then submitted with Poisson arrivals at 70 % of
               it: p50/p99 latency and cuboids/s; every job's counts are checked
               against the sum of its subtrees' (a sequential binary64 run)
@@ -82,7 +83,7 @@ def parse():
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight in the batched legs")
     ap.add_argument("--batch-jobs", type=int, default=16, help="proofs per batch (one grid row each)")
-    ap.add_argument("--stream-batch-jobs", type=int, default=8,
+    ap.add_argument("--stream-batch-jobs", type=int, default=4,
                     help="jobs per batch in the job-stream leg (11 programs share the memory budget)")
     ap.add_argument("--batched-proofs", type=int, default=1024)
     ap.add_argument("--stream-jobs", type=int, default=48)
@@ -285,7 +286,7 @@ def job_stream(sqd, jobs_n, workers, batch_jobs, seed):
     with sqd.Service(workers=1, fp32_prefilter=0) as svc:
         for j in set(jobs):
             alone[j] = svc.wait(svc.submit(progs[j[0]], j[1], j[2]))["counts"]
-    with sqd.Service(workers=workers, autotune=1, batch_jobs=batch_jobs) as svc:
+    with sqd.Service(workers=workers, autotune=1, batch_jobs=batch_jobs, memory_budget=64 << 30) as svc:
         for k in LEMMAS:                                      # tune and create a core per statement
             svc.wait(svc.submit(progs[k], 0, 1))
         t0 = time.perf_counter()
@@ -312,7 +313,8 @@ def job_stream(sqd, jobs_n, workers, batch_jobs, seed):
     lat = [r["latency_seconds"] for r in res]
     cub = sum(r["counts"]["num_cuboids"] for r in res)
     return {
-        "jobs": len(jobs), "seed": seed, "workers": workers, "batch_jobs": batch_jobs, "arrival_rate_per_s": rate,
+        "jobs": len(jobs), "seed": seed, "workers": workers, "batch_jobs": batch_jobs, "memory_budget_gib": 64,
+        "arrival_rate_per_s": rate,
         "capacity_jobs_per_s": len(jobs) / makespan,
         "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3},
         "cuboids_per_s": cub / elapsed, "cuboids": cub, "seconds": elapsed,
