@@ -133,37 +133,26 @@ __global__ void k_round_begin(DeviceState* st, unsigned long long k_max, unsigne
     if(t < 32) st->undec_count[t] = 0;
 }
 // Adds the per-block count slots into the state's counters and clears them
-// (one block of kFoldThreads per job; run before the state is read).  Each
-// thread loads up to four slots before it clears any, so that the loads are in
-// flight together, then the sums go through wave reductions.
-constexpr unsigned kFoldThreads = 1024;
+// (run before the state is read).  Each block folds kFoldThreads slots of its
+// job (one per thread), reduces them across its waves and adds the sums to the
+// counters with atomics, so that the slots are read by many compute units at
+// once instead of one.
+constexpr unsigned kFoldThreads = 256;
 __global__ __launch_bounds__(kFoldThreads) void k_fold_counts(DeviceState* st, unsigned long long* slots,
                                                               unsigned long long n_slots, unsigned long long stride) {
     st = dev::job_ptr(st, dev::job_offset(stride));
     slots = dev::job_ptr(slots, dev::job_offset(stride));
     constexpr int W = 5;   // used words of a slot
-    constexpr int U = 4;   // slots a thread loads at once
     __shared__ unsigned long long part[W][kFoldThreads / 64];
     unsigned long long acc[W] = {0, 0, 0, 0, 0};
-    for(unsigned long long i0 = threadIdx.x; i0 < n_slots; i0 += U * static_cast<unsigned long long>(blockDim.x)) {
-        unsigned long long v[U][W];
+    for(unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_slots;
+        i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+        unsigned long long* s = slots + static_cast<unsigned long long>(dev::kSlotWords) * i;
+        unsigned long long v[W];
 #pragma unroll
-        for(int u = 0; u < U; ++u) {
-            const unsigned long long i = i0 + static_cast<unsigned long long>(u) * blockDim.x;
-            const unsigned long long* s = slots + static_cast<unsigned long long>(dev::kSlotWords) * i;
+        for(int j = 0; j < W; ++j) v[j] = s[j];
 #pragma unroll
-            for(int j = 0; j < W; ++j) v[u][j] = i < n_slots ? s[j] : 0ull;
-        }
-#pragma unroll
-        for(int u = 0; u < U; ++u) {
-            const unsigned long long i = i0 + static_cast<unsigned long long>(u) * blockDim.x;
-            unsigned long long* s = slots + static_cast<unsigned long long>(dev::kSlotWords) * i;
-#pragma unroll
-            for(int j = 0; j < W; ++j) {
-                acc[j] += v[u][j];
-                if(i < n_slots) s[j] = 0ull;
-            }
-        }
+        for(int j = 0; j < W; ++j) { acc[j] += v[j]; s[j] = 0ull; }
     }
 #pragma unroll
     for(int j = 0; j < W; ++j) acc[j] = dev::wave_sum(acc[j]);
@@ -174,11 +163,11 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_counts(DeviceState* st, u
         unsigned long long sum[W] = {0, 0, 0, 0, 0};
         for(unsigned w = 0; w < blockDim.x / 64; ++w)
             for(int j = 0; j < W; ++j) sum[j] += part[j][w];
-        st->counters.cuboids += sum[0];
-        st->counters.leaves += sum[1];
-        st->counters.criticals += sum[2];
-        st->f32_decided += sum[3];
-        st->f32_checked += sum[4];
+        if(sum[0]) atomicAdd(&st->counters.cuboids, sum[0]);
+        if(sum[1]) atomicAdd(&st->counters.leaves, sum[1]);
+        if(sum[2]) atomicAdd(&st->counters.criticals, sum[2]);
+        if(sum[3]) atomicAdd(&st->f32_decided, sum[3]);
+        if(sum[4]) atomicAdd(&st->f32_checked, sum[4]);
     }
 }
 // Level counters to zero (between chunks of a host-driven call).
@@ -415,7 +404,8 @@ void reset_state(sqd_core* c) {
 // device state (of every job of the launches' rows) to its pinned host copy.
 void enqueue_state_copy(sqd_core* c) {
     launch(c, KIND_SET_COUNT, "k_fold_counts", [&] {
-        hipLaunchKernelGGL(k_fold_counts, dim3(1, c->ny), dim3(kFoldThreads), 0, c->stream, c->d_state, c->d_slots, c->n_slots,
+        const unsigned blocks = static_cast<unsigned>((c->n_slots + kFoldThreads - 1) / kFoldThreads);
+        hipLaunchKernelGGL(k_fold_counts, dim3(blocks, c->ny), dim3(kFoldThreads), 0, c->stream, c->d_state, c->d_slots, c->n_slots,
                            c->job_stride);
         return hipGetLastError();
     });
