@@ -630,9 +630,11 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
 // parent (cpl from args.children_per_lane or the launch size, at most CPL) and
 // checks the rows once on the parent's box with A over the whole group, the
 // terms free of A evaluated once (P::hoist).  The outward-rounded operations are
-// inclusion-isotone (each bound is the directed rounding of the exact bound, a
-// monotone function of the operands' bounds), and every child's hybrid box lies
-// in the group's box, so a group that fails a row holds only children that the
+// inclusion-isotone where defined (each bound is the directed rounding of the
+// exact bound, a monotone function of the operands' bounds; a sub-box is
+// possibly undefined only where the box is, and a comparison with a possibly
+// undefined operand is never definite), and every child's hybrid box lies in
+// the group's box, so a group that fails a row holds only children that the
 // per-child test prunes too: they are counted as pruned by it.  The children of
 // the other groups are gathered in LDS and spread over the block's lanes for
 // the per-child test (propagation, then the rows on the hybrid box); the
@@ -756,8 +758,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
 // cpl consecutive children (the terms free of A hoisted), and appends the
 // children of a group that passes to args.undec; k_static_level<..., true> then
 // decides them one by one exactly as the plain pass does.  The binary64
-// operations are inclusion-isotone and every child's propagated box lies in the
-// group's box, so a group that fails a row holds only children that the plain
+// operations are inclusion-isotone where defined (as for the FP32 prefilter's
+// group test) and every child's propagated box lies in the group's box, so a group that fails a row holds only children that the plain
 // pass prunes too (by propagation or by a row): they are counted as its pruned
 // leaves, and survivors, hulls, criticals and counts are exactly the plain
 // pass's.  The test runs in its own kernel so that neither pass carries the
