@@ -666,6 +666,13 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     const unsigned long long trips = (total + stride - 1) / stride;
     __shared__ unsigned long long s_groups[kBlock];   // the trip's groups that the group test left
     __shared__ unsigned long long s_ngroups;
+    // The hoisted terms of the parents of one block trip (at most MAXP of them:
+    // a trip's kBlock consecutive groups, at least SUBS / CPL per parent), kept
+    // for the per-child test when the table fits in 16 KiB.
+    using H = typename P::template Hoist<L, float>;
+    constexpr int MAXP = kBlock / (SUBS / CPL) + 2;
+    constexpr bool SHARE = sizeof(H) > 1 && MAXP * sizeof(H) <= 16384;
+    __shared__ H s_h[SHARE ? MAXP : 1];
     unsigned long long c_cub = 0, c_leaf = 0, c_dec = 0, c_chk = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
         if(threadIdx.x == 0) s_ngroups = 0;
@@ -673,6 +680,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
         // 1. group test: the lane's group is children [g * cpl, (g + 1) * cpl) of
         //    one parent; the rows are checked once on the parent's box with the
         //    level's variable over the whole group
+        const unsigned long long s0 = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + trip * stride;
+        const unsigned long long item0 = s0 / static_cast<unsigned long long>(step);   // the trip's first parent
         const unsigned long long slot = first + trip * stride;
         bool keep = false;
         if(slot < total) {
@@ -680,22 +689,27 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
             const int g = static_cast<int>(slot % static_cast<unsigned long long>(step));
             if(g == 0) c_cub += SUBS;
             keep = true;
-            if(cpl > 1) {
+            if(cpl > 1 || SHARE) {
                 double lb[N], ub[N];
                 load_box<P>(args.in, in_base + item, lb, ub);
                 float hl[N], hu[N];
 #pragma unroll
                 for(int a = 0; a < N; ++a) { hl[a] = f32_down(lb[a]); hu[a] = f32_up(ub[a]); }
-                typename P::template Hoist<L, float> h;
+                H h;
                 P::template hoist<L>(hl, hu, s_c32, h);
-                const double wd = split_width(lb[ARG], ub[ARG], SUBS);
-                hl[ARG] = f32_down(split_point(lb[ARG], ub[ARG], wd, g * cpl, SUBS));
-                hu[ARG] = f32_up(split_point(lb[ARG], ub[ARG], wd, (g + 1) * cpl, SUBS));
-                if(P::template rows_violated_hoisted<L>(hl, hu, s_c32, h)) {
-                    keep = false;
-                    c_chk += static_cast<unsigned long long>(cpl);
-                    c_leaf += static_cast<unsigned long long>(cpl);
-                    c_dec += static_cast<unsigned long long>(cpl);
+                if constexpr(SHARE) {
+                    if(g == 0 || slot == s0) s_h[item - item0] = h;   // one lane per parent
+                }
+                if(cpl > 1) {
+                    const double wd = split_width(lb[ARG], ub[ARG], SUBS);
+                    hl[ARG] = f32_down(split_point(lb[ARG], ub[ARG], wd, g * cpl, SUBS));
+                    hu[ARG] = f32_up(split_point(lb[ARG], ub[ARG], wd, (g + 1) * cpl, SUBS));
+                    if(P::template rows_violated_hoisted<L>(hl, hu, s_c32, h)) {
+                        keep = false;
+                        c_chk += static_cast<unsigned long long>(cpl);
+                        c_leaf += static_cast<unsigned long long>(cpl);
+                        c_dec += static_cast<unsigned long long>(cpl);
+                    }
                 }
             }
         }
@@ -728,11 +742,17 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
                 ++c_chk;
                 bool pruned = propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop);
                 if(!pruned) {
-                    typename P::template Hoist<L, float> h;
-                    P::template hoist<L>(plb, pub, s_c32, h);
-                    plb[ARG] = lb32[ARG];
-                    pub[ARG] = ub32[ARG];
-                    pruned = P::template rows_violated_hoisted<L>(plb, pub, s_c32, h);
+                    if constexpr(SHARE) {
+                        plb[ARG] = lb32[ARG];
+                        pub[ARG] = ub32[ARG];
+                        pruned = P::template rows_violated_hoisted<L>(plb, pub, s_c32, s_h[item - item0]);
+                    } else {
+                        H h;
+                        P::template hoist<L>(plb, pub, s_c32, h);
+                        plb[ARG] = lb32[ARG];
+                        pub[ARG] = ub32[ARG];
+                        pruned = P::template rows_violated_hoisted<L>(plb, pub, s_c32, h);
+                    }
                 }
                 if(pruned) {
                     ++c_leaf;
