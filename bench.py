@@ -204,7 +204,8 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
         tune = {"settings": {f: getattr(best, f) for f, _ in sqd.Settings._fields_}, "first": None, "last": None}
     else:
         best, tune = sqd.autotune(prog, base=base)        # as the service tunes it (reported)
-    tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "prefilter_group", "dequeue_buffer_size", "grid_blocks")}
+    tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "group_levels", "prefilter_group", "dequeue_buffer_size",
+                                              "grid_blocks")}
     n = initial_subdivisions(prog)
     # decided fraction and the dominant prefiltered kernel's time on one core
     one = sqd.Settings.from_buffer_copy(best)
