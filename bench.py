@@ -203,7 +203,7 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
             setattr(best, k.strip(), int(v, 0))
         tune = {"settings": {f: getattr(best, f) for f, _ in sqd.Settings._fields_}, "first": None, "last": None}
     else:
-        best, tune = sqd.autotune(prog, base=base)        # as the service tunes it (reported)
+        best, tune = sqd.autotune(prog, base=base, fp32_first=True)   # on batches, as the service runs them
     tuned = {k: tune["settings"][k] for k in ("fp32_prefilter", "group_levels", "prefilter_group", "dequeue_buffer_size",
                                               "grid_blocks")}
     n = initial_subdivisions(prog)

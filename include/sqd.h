@@ -258,7 +258,9 @@ typedef struct sqd_autotune_options {
     double target_seconds;    /* slice of the first variable to time: about this long, and about
                                  this much timing per candidate (default 0.02) */
     int32_t repetitions;      /* at least this many timed proofs per candidate, median taken (default 3) */
-    uint32_t reserved;
+    uint32_t fp32_first;      /* nonzero: the binary64 group test is tried only on the levels the
+                                 best prefilter mask leaves; 0 (default): also on its own, without
+                                 the prefilter (exact binary64 results) */
 } sqd_autotune_options;
 typedef struct sqd_autotune_report {
     uint32_t first, last;     /* the slice [first, last) that was timed */

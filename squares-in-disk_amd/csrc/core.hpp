@@ -548,7 +548,12 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
     const LevelArgs args = job_args(job0);
     constexpr int SUBS = P::st_subs(L);
     const unsigned long long n_items = *args.in_count;
-    const unsigned long long total = F32 ? *args.undec_count : n_items * static_cast<unsigned long long>(SUBS);
+    // the list holds at most undec_cap entries; a pass that found more counted
+    // them (and raised the overflow flag, which fails the call) but stored only
+    // those that fit
+    const unsigned long long listed = F32 ? *args.undec_count : 0ull;
+    const unsigned long long total =
+        F32 ? (listed < args.undec_cap ? listed : args.undec_cap) : n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if(F32 && static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // block-uniform
     __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];

@@ -899,6 +899,15 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         unsigned long long kmax = s.dequeue_buffer_size ? s.dequeue_buffer_size : 8192;
         kmax = std::min(kmax, lvl_cap / worst_static);
         kmax = std::min(kmax, lvl_cap / worst_split);
+        // binary64 group test: the children of the groups it leaves go to the
+        // undecided list (as many entries as a level buffer holds boxes), up to
+        // all children of the level, so a round is bounded by the largest such
+        // level's children per dynamic cuboid
+        unsigned long long worst_group = 1;
+        for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L)
+            if(((s.group_levels & ~s.fp32_prefilter) >> L) & 1u)
+                worst_group = std::max<unsigned long long>(worst_group, product(c->st_subs, L + 1));
+        kmax = std::min(kmax, lvl_cap / worst_group);
         if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
         c->k_max = kmax;
         // count slots: one per block of the largest counting grid (level kernels

@@ -192,13 +192,20 @@ int sqd_autotune(int device, const sqd_program* prog, const sqd_settings* base, 
             s.fp32_prefilter = m;
             consider(s);
         }
-        // 2. binary64 group test on the levels the prefilter does not take
+        // 2. binary64 group test on the levels the prefilter does not take, and
+        //    (unless fp32_first) on its own, without the prefilter
         const sqd_settings s_grp = cur.s;
         for(uint32_t m = 1; m < masks; ++m) {
-            if(m & s_grp.fp32_prefilter) continue;
-            sqd_settings s = s_grp;
-            s.group_levels = m;
-            consider(s);
+            if(s_grp.fp32_prefilter && !(m & s_grp.fp32_prefilter)) {
+                sqd_settings s = s_grp;
+                s.group_levels = m;
+                consider(s);
+            }
+            if(!o.fp32_first || !s_grp.fp32_prefilter) {
+                sqd_settings s = s_mask;
+                s.group_levels = m;
+                consider(s);
+            }
         }
         // 3. children per group (when a level groups them)
         if(cur.s.fp32_prefilter || cur.s.group_levels) {

@@ -50,7 +50,7 @@ class WidthStat(ctypes.Structure):
 
 
 class AutotuneOptions(ctypes.Structure):
-    _fields_ = [("target_seconds", ctypes.c_double), ("repetitions", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
+    _fields_ = [("target_seconds", ctypes.c_double), ("repetitions", ctypes.c_int32), ("fp32_first", ctypes.c_uint32)]
 
 
 class AutotuneReport(ctypes.Structure):
@@ -485,12 +485,13 @@ def interval_op(op, a, b=None, device=0):
     return out
 
 
-def autotune(program, device=0, base=None, target_seconds=0.02, repetitions=3):
+def autotune(program, device=0, base=None, target_seconds=0.02, repetitions=3, fp32_first=False):
     """sqd_autotune: the best settings found and the report (dict)."""
     o = AutotuneOptions()
     lib().sqd_autotune_options_default(ctypes.byref(o))
     o.target_seconds = target_seconds
     o.repetitions = repetitions
+    o.fp32_first = 1 if fp32_first else 0
     best = Settings()
     rep = AutotuneReport()
     _check(lib().sqd_autotune(device, program._h, ctypes.byref(base) if base is not None else None, ctypes.byref(o),

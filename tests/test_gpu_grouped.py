@@ -71,3 +71,30 @@ def test_grouped_with_prefilter_and_batches():
         got = core.prove_batch([(0, n), (0, n // 2), (n // 2, n)])
     assert (got[0]["counts"]["num_cuboids"], got[0]["counts"]["num_leaf_cuboids"]) == PUBLISHED["one"]
     assert got[1]["counts"]["num_cuboids"] + got[2]["counts"]["num_cuboids"] == PUBLISHED["one"][0]
+
+
+def test_grouped_small_buffers_exact():
+    # small level buffers and the largest round size: the core shrinks its rounds
+    # so that a grouped level's children always fit the undecided list
+    p = sqd.Program.from_lemma("two")
+    s = sqd.default_settings(group_levels=ALL, prefilter_group=64, level_bytes=4 << 20, dequeue_buffer_size=65536)
+    with sqd.Core(p, settings=s) as core:
+        r = core.prove()
+    assert (r["counts"]["num_cuboids"], r["counts"]["num_leaf_cuboids"]) == PUBLISHED["two"]
+
+
+def test_prefilter_small_buffers_exact_or_capacity_error():
+    # the FP32 prefilter's undecided list may overflow with small buffers: the
+    # call then fails with a capacity error (the binary64 pass reads only the
+    # entries that were stored), otherwise the counts are the published ones
+    p = sqd.Program.from_lemma("two")
+    s = sqd.default_settings(fp32_prefilter=sqd.FP32_ALL_LEVELS, level_bytes=4 << 20, dequeue_buffer_size=65536)
+    with sqd.Core(p, settings=s) as core:
+        try:
+            r = core.prove()
+        except sqd.SqdError as e:
+            assert "overflow" in str(e) or "capacity" in str(e)
+        else:
+            assert (r["counts"]["num_cuboids"], r["counts"]["num_leaf_cuboids"]) == PUBLISHED["two"]
+        r = core.prove_range(5, 6)                     # the core stays usable
+        assert r["counts"]["num_cuboids"] > 0
