@@ -295,7 +295,8 @@ typedef struct sqd_service_options {
     uint32_t autotune;            /* nonzero: sqd_autotune each program on its first job */
     uint64_t level_bytes;         /* per core (default 256 MiB) */
     uint64_t frontier_capacity;   /* per core (default 1 << 22 cuboids) */
-    uint64_t core_bytes_estimate; /* size assumed for a program's first core (default 2 GiB) */
+    uint64_t core_bytes_estimate; /* size assumed per job of a program's first core, before one
+                                     has been measured (default 2 GiB) */
     uint32_t batch_jobs;          /* queued jobs of one program a worker proves together
                                      (sqd_prove_batch; each core holds that many; default 8) */
     uint32_t reserved;

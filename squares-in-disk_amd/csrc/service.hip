@@ -307,7 +307,8 @@ struct sqd_service {
                     s.busy = true;
                     return s.id;
                 }
-            const uint64_t need = core_bytes.count(prog) ? core_bytes[prog] : opt.core_bytes_estimate;
+            const uint64_t need = core_bytes.count(prog) ? core_bytes[prog]
+                                                         : opt.core_bytes_estimate * opt.batch_jobs;
             if(bytes_in_use + need <= opt.memory_budget || pool.empty()) {
                 pool.push_back(Slot{next_slot++, prog, nullptr, need, true});
                 bytes_in_use += need;
