@@ -17,7 +17,11 @@ checked against the published ones.  The line printed carries
               algorithmic binary64 operations per launch (op-counted
               restatement of the same algorithm, per kernel kind) over its
               mean launch time from HIP events on the core's stream, against
-              the MI355X FP64 vector peak
+              the MI355X FP64 vector peak; with the same mean launch time
+              against the least time its VALU wave instructions (a
+              SQ_INSTS_VALU count per launch from the counter summary that
+              also gives the HBM traffic) need to issue, the bound that
+              limits this interval kernel
   cpu_baseline the CPU restatement (oracle) timed on this host's cores on the
               same proof
   fp32_first_batched
@@ -54,6 +58,7 @@ sys.path.insert(0, str(ROOT / "squares-in-disk_amd"))
 
 METRIC = "cuboids/sec + p50/p99 proof-job latency, FP32-first batched vs plain FP64, 1 GPU"
 FP64_PEAK_TFLOPS = 78.6          # MI355X binary64 vector peak
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4   # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU op holds a SIMD 4 cycles
 PUBLISHED = {                    # (cuboids, leaves) from the reference's prover output
     "top1": (128, 128), "top2": (168, 168), "one": (2869888, 2858678), "two": (7181696, 7069587),
     "three": (3549856, 3440786), "four": (37245328, 35026303), "five": (3392, 3188), "six": (14848, 13928),
@@ -163,8 +168,8 @@ def fp32_op_counts(program_text):
         os.unlink(path)
 
 
-def read_traffic(path, kernel_name):
-    """Bytes per launch of `kernel_name` from a traffic summary (key = value lines)."""
+def read_counters(path, kernel_name):
+    """The key = value lines of a counter summary of `kernel_name` (None if it covers another kernel)."""
     if not path or not Path(path).exists():
         return None
     vals = {}
@@ -172,9 +177,22 @@ def read_traffic(path, kernel_name):
         if "=" in line and not line.lstrip().startswith("#"):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
-    if vals.get("kernel") != kernel_name or "bytes_per_launch" not in vals:
+    return vals if vals.get("kernel") == kernel_name else None
+
+
+def read_traffic(path, kernel_name):
+    """Bytes per launch of `kernel_name` from a traffic summary."""
+    vals = read_counters(path, kernel_name)
+    return float(vals["bytes_per_launch"]) if vals and "bytes_per_launch" in vals else None
+
+
+def valu_issue_bound_ms(path, kernel_name):
+    """The least time the launch's VALU wave instructions (SQ_INSTS_VALU) need at one
+    issue per SIMD every 4 cycles on all 1024 SIMDs at 2.4 GHz."""
+    vals = read_counters(path, kernel_name)
+    if not vals or "valu_wave_instructions_per_launch" not in vals:
         return None
-    return float(vals["bytes_per_launch"])
+    return float(vals["valu_wave_instructions_per_launch"]) * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e3
 
 
 LEMMAS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven", "more1", "more2"]
@@ -385,6 +403,7 @@ def main():
         flops_per_launch = flops_per_proof / launches_per_proof
         achieved = flops_per_launch / (mean_ms * 1e-3) / 1e12
         traffic = read_traffic(a.traffic, dom["name"])
+        issue_ms = valu_issue_bound_ms(a.traffic, dom["name"])
         roofline = {
             "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
@@ -392,6 +411,8 @@ def main():
             "flops_per_launch": flops_per_launch,
             "cuboids_per_launch": cuboids_per_proof / launches_per_proof,
             "flops_per_cuboid": flops_per_proof / cuboids_per_proof if cuboids_per_proof else None,
+            "valu_issue_bound_ms": issue_ms,
+            "valu_issue_frac": issue_ms / mean_ms if issue_ms else None,
         }
     line = {
         "metric": METRIC,
