@@ -197,7 +197,8 @@ struct Emitter {
         switch(n.op) {
             case FOp::Const: {
                 int c = cidx(id);
-                o << "const IV " << v(id) << "{C[" << 2 * c << "], C[" << 2 * c + 1 << "], " << (n.c_undef ? "true" : "false") << "};";
+                o << "const IV " << v(id) << "{kc<Num, " << 2 * c << ">(C), kc<Num, " << 2 * c + 1 << ">(C), "
+                  << (n.c_undef ? "true" : "false") << "};";
                 break;
             }
             case FOp::Arg: o << "const IV " << v(id) << "{lb[" << n.arg << "], ub[" << n.arg << "], false};"; break;
@@ -445,6 +446,16 @@ std::string generate(const flat::Program& p, const std::string& key) {
     o << "    static constexpr int NC = " << e.consts.size() / 2 << ";\n";
     o << "    static constexpr int NDYN = " << dyn.size() << ";\n";
     o << "    static constexpr int NSTAT = " << sta.size() << ";\n";
+    // binary64 constants as literals (folded into the instructions that use
+    // them); binary32 code reads the outward-converted copy in LDS
+    o << "    static constexpr double K[] = {";
+    for(std::size_t i = 0; i < e.consts.size(); ++i) o << (i ? ", " : "") << dlit(e.consts[i]);
+    if(e.consts.empty()) o << "0.0, 0.0";
+    o << "};\n";
+    o << "    template<class Num, int I> __device__ static __forceinline__ Num kc(const Num* C) {\n"
+         "        if constexpr(sizeof(Num) == sizeof(double)) { constexpr double k = K[I]; (void)C; return k; }\n"
+         "        else { return C[I]; }\n"
+         "    }\n";
     auto table = [&](const char* name, const std::vector<int>& idx, auto get) {
         o << "    static constexpr int " << name << "(int j) { switch(j) {";
         for(std::size_t j = 0; j < idx.size(); ++j) o << " case " << j << ": return " << get(p.splits[static_cast<std::size_t>(idx[j])]) << ";";
