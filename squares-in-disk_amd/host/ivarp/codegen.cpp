@@ -193,6 +193,15 @@ struct Emitter {
         const flat::FNode& n = p.nodes[static_cast<std::size_t>(id)];
         auto K = [&](std::size_t i) { return n.kids[i]; };
         auto KN = [&](std::size_t i) -> const flat::FNode& { return p.nodes[static_cast<std::size_t>(n.kids[i])]; };
+        // Binary64 constants are literals, and the compiler folds operations on
+        // literals alone in round-to-nearest, not in the directed rounding the
+        // kernels run in: such terms must have been folded exactly by lowering.
+        if(n.op != FOp::Const && n.op != FOp::Arg && !n.kids.empty()) {
+            bool all_const = true;
+            for(std::size_t i = 0; i < n.kids.size(); ++i) all_const = all_const && KN(i).op == FOp::Const;
+            if(all_const)
+                throw std::runtime_error("codegen: node " + std::to_string(id) + " operates on constants only");
+        }
         std::ostringstream o;
         switch(n.op) {
             case FOp::Const: {
