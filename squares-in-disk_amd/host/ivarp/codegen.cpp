@@ -13,6 +13,7 @@
 #include <set>
 #include <utility>
 #include <vector>
+#include <cmath>
 #include <cstdio>
 
 namespace ivarp {
@@ -27,6 +28,22 @@ std::string dlit(double d) {
     if(d == -__builtin_huge_val()) return "(-__builtin_huge_val())";
     char buf[64];
     std::snprintf(buf, sizeof buf, "%a", d);
+    return buf;
+}
+// The largest float <= d (up: the smallest float >= d), as the kernels'
+// outward conversion gives it; the conversion here rounds to nearest and the
+// comparison with the exact widening corrects it, zero signs included.
+float f32_outward(double d, bool up) {
+    float f = static_cast<float>(d);
+    if(!up && static_cast<double>(f) > d) f = std::nextafter(f, -__builtin_huge_valf());
+    if(up && static_cast<double>(f) < d) f = std::nextafter(f, __builtin_huge_valf());
+    return f;
+}
+std::string flit(float f) {
+    if(f == __builtin_huge_valf()) return "__builtin_huge_valf()";
+    if(f == -__builtin_huge_valf()) return "(-__builtin_huge_valf())";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%af", static_cast<double>(f));
     return buf;
 }
 int tri_nn(const flat::FNode& b) { return b.lb >= 0 ? 1 : (b.ub <= 0 ? 0 : 2); }
@@ -455,15 +472,20 @@ std::string generate(const flat::Program& p, const std::string& key) {
     o << "    static constexpr int NC = " << e.consts.size() / 2 << ";\n";
     o << "    static constexpr int NDYN = " << dyn.size() << ";\n";
     o << "    static constexpr int NSTAT = " << sta.size() << ";\n";
-    // binary64 constants as literals (folded into the instructions that use
-    // them); binary32 code reads the outward-converted copy in LDS
+    // constants as literals (folded into the instructions that use them): the
+    // binary64 intervals, and the same converted outward to binary32
     o << "    static constexpr double K[] = {";
     for(std::size_t i = 0; i < e.consts.size(); ++i) o << (i ? ", " : "") << dlit(e.consts[i]);
     if(e.consts.empty()) o << "0.0, 0.0";
     o << "};\n";
+    o << "    static constexpr float K32[] = {";
+    for(std::size_t i = 0; i < e.consts.size(); ++i) o << (i ? ", " : "") << flit(f32_outward(e.consts[i], i & 1));
+    if(e.consts.empty()) o << "0.0f, 0.0f";
+    o << "};\n";
     o << "    template<class Num, int I> __device__ static __forceinline__ Num kc(const Num* C) {\n"
-         "        if constexpr(sizeof(Num) == sizeof(double)) { constexpr double k = K[I]; (void)C; return k; }\n"
-         "        else { return C[I]; }\n"
+         "        (void)C;\n"
+         "        if constexpr(sizeof(Num) == sizeof(double)) { constexpr double k = K[I]; return k; }\n"
+         "        else { constexpr float k = K32[I]; return k; }\n"
          "    }\n";
     auto table = [&](const char* name, const std::vector<int>& idx, auto get) {
         o << "    static constexpr int " << name << "(int j) { switch(j) {";
