@@ -556,9 +556,10 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArg
         F32 ? (listed < args.undec_cap ? listed : args.undec_cap) : n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if(F32 && static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // block-uniform
-    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    // the generated code takes the constants as literals (P::kc) and does not
+    // read the array pointer its functions take
+    const double* const s_c = nullptr;
     __shared__ PropScratch<P> s_prop;
-    load_consts<P>(s_c, args.consts);
     enter_round_down();
     const unsigned long long in_base = input_base(args);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
@@ -655,11 +656,10 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
     constexpr int CPL = children_per_lane<SUBS>();   // most children per lane
-    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
-    __shared__ float s_c32[2 * (P::NC > 0 ? P::NC : 1)];
+    // constants are literals in the generated code (P::kc, binary32 converted
+    // outward); the array pointer its functions take is not read
+    const float* const s_c32 = nullptr;
     __shared__ PropScratch<P> s_prop;
-    load_consts<P>(s_c, args.consts);
-    load_consts_f32<P>(s_c32, s_c);
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
@@ -796,8 +796,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_group64(LevelA
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
     constexpr int CPL = children_per_lane<SUBS>();
-    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
-    load_consts<P>(s_c, args.consts);
+    const double* const s_c = nullptr;   // constants are literals in the generated code (P::kc)
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
@@ -857,9 +856,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
     constexpr int SUBS = MODE == INIT ? P::dyn_initial(J) : P::dyn_subs(J);
     constexpr bool LAST = (J == P::NDYN - 1);
     constexpr unsigned REC = MODE == INIT ? (N < 255 ? N : 255) : (N < 32 ? N : 32);
-    __shared__ double s_c[2 * (P::NC > 0 ? P::NC : 1)];
+    const double* const s_c = nullptr;   // constants are literals in the generated code (P::kc)
     __shared__ PropScratch<P> s_prop;
-    load_consts<P>(s_c, args.consts);
     enter_round_down();
     const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
