@@ -45,6 +45,12 @@ constexpr int kBlock = 256;
 #else
 #define SQD_LEVEL_ATTR
 #endif
+// The binary64 static level kernels take their program's budget for the level
+// (P::level_waves(L); 0 or 1: no minimum) unless SQD_LEVEL_WAVES_PER_EU is set.
+template<class P, int L>
+constexpr int static_level_waves() {
+    return SQD_LEVEL_WAVES_PER_EU > 0 ? SQD_LEVEL_WAVES_PER_EU : (P::level_waves(L) > 0 ? P::level_waves(L) : 1);
+}
 
 enum Mode : int { NONFINAL = 0, FINAL = 1, SPLIT = 2, INIT = 3 };
 // Hoisted-term struct of a program without static levels.
@@ -544,7 +550,8 @@ constexpr bool wave_hoisting() {
 // binary32 left undecided; by k_static_group64, those of the groups that passed
 // the group test) are decided in binary64, exactly as above.
 template<class P, int MODE, int L, bool F32>
-__global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_level(LevelArgs job0) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(static_level_waves<P, L>())))
+void k_static_level(LevelArgs job0) {
     const LevelArgs args = job_args(job0);
     constexpr int SUBS = P::st_subs(L);
     const unsigned long long n_items = *args.in_count;

@@ -497,6 +497,15 @@ std::string generate(const flat::Program& p, const std::string& key) {
     table("dyn_subs", dyn, [](const flat::FSplit& s) { return static_cast<int>(s.subs); });
     table("st_arg", sta, [](const flat::FSplit& s) { return s.arg; });
     table("st_subs", sta, [](const flat::FSplit& s) { return static_cast<int>(s.subs); });
+    // Register budget of the binary64 static level kernels as resident waves
+    // per SIMD (0: the compiler's choice).  Measured per kernel: a budget of 4
+    // (128 VGPRs, a few spilled) made One's and Two's level 1 and More2's
+    // level 2 faster and Four's level 2 slower.
+    int waves_l1 = 0, waves_l2 = 0;
+    if(key == "one" || key == "two") waves_l1 = 4;
+    if(key == "more2") waves_l2 = 4;
+    o << "    static constexpr int level_waves(int l) { return l == 1 ? " << waves_l1 << " : l == 2 ? " << waves_l2
+      << " : 0; }\n";
     o << body.str();
     o << "};\n\n";
     o << "static const double k_consts[] = {";
