@@ -57,7 +57,9 @@ typedef struct sqd_settings {
     uint32_t dequeue_buffer_size;  /* dynamic cuboids per round; default chosen per program */
     uint64_t frontier_capacity;    /* cuboids the device frontier can hold; default 1<<23 */
     uint64_t level_bytes;          /* bytes per level buffer; default 1<<30 */
-    uint64_t critical_capacity;    /* criticals kept for the caller; default 1<<20 */
+    uint64_t critical_capacity;    /* criticals the device list holds (default 1<<20; at least the
+                                      last static level's subdivisions): more fail a call unless a
+                                      critical sink receives them (sqd_core_set_critical_sink) */
     uint32_t grid_blocks;          /* blocks per level launch; default: the kernel's resident
                                       capacity (occupancy x CUs), capped by the work */
     uint32_t fp32_prefilter;       /* bit mask of the static levels (bit L: level L) that first
@@ -131,7 +133,9 @@ int sqd_nonfinal(sqd_core* core, const double* lb, const double* ub, size_t n,
                  double* hull_lb, double* hull_ub, uint8_t* has_hull, sqd_counts* counts);
 
 /* handle_cuboids_final (cpu_prover_core.hpp:62-71): critical leaves of n boxes,
- * up to `capacity` (SoA, stride capacity); *n_critical receives the total. */
+ * up to `capacity` (SoA, stride capacity); *n_critical receives the total (the
+ * rest can be fetched with sqd_core_criticals).  With a critical sink set, every
+ * critical goes to the sink instead and none to crit_lb/crit_ub. */
 int sqd_final(sqd_core* core, const double* lb, const double* ub, size_t n,
               double* crit_lb, double* crit_ub, size_t capacity, size_t* n_critical, sqd_counts* counts);
 
@@ -183,6 +187,24 @@ int sqd_core_job_criticals(sqd_core* core, size_t job, double* crit_lb, double* 
  * capacity); *n receives the total found.  The reference hands each critical
  * to its on_critical callback (prover/proof_driver.hpp:499-509). */
 int sqd_core_criticals(sqd_core* core, double* crit_lb, double* crit_ub, size_t capacity, size_t* n);
+
+/* on_critical (set_on_critical, prover/basic_prover_core.hpp:30-67; called per
+ * critical in cpu_prover/cpu_prover_core.hpp:137-149; the reference's GPU core
+ * re-launches until its output buffer holds every critical,
+ * with_cuda/ivarp_cuda/gpu_prover_core.cuh:293-316,
+ * cuda_output_buffer.cuh:68-107): with a sink set, sqd_prove, sqd_prove_range
+ * and sqd_final on a core of one job hand every critical cuboid to `fn` as the
+ * proof finds them, in chunks of n boxes in SoA layout (lb[a * stride + i]),
+ * valid during the call, on the calling thread.  The device critical list is
+ * drained after every round; a round with more criticals than
+ * critical_capacity repeats its final pass in parts that fit (counting nothing
+ * twice), so no critical is lost.  Those calls then copy no criticals to their
+ * own output arrays and report the total in *n_critical.  Without a sink, more
+ * criticals than critical_capacity fail the call with SQD_ERR_CAPACITY.  Batches
+ * of several jobs keep up to critical_capacity criticals per job.  fn = NULL
+ * removes the sink. */
+typedef void (*sqd_critical_fn)(void* user, const double* lb, const double* ub, size_t n, size_t stride);
+int sqd_core_set_critical_sink(sqd_core* core, sqd_critical_fn fn, void* user);
 
 /* Progress of sqd_prove / sqd_prove_range: the reference's ProgressInfo and
  * observer calls (prover/proof_driver.hpp:551-603, progress_printer.hpp:47-90):
@@ -237,6 +259,20 @@ typedef struct sqd_width_stat {
 } sqd_width_stat;
 int sqd_width_stats(sqd_core* core, const double* lb, const double* ub, size_t n, sqd_width_stat* out,
                     size_t capacity, size_t* n_constraints);
+
+/* Values of the program's expressions on n boxes (SoA, stride n), in binary64
+ * (precision 64) or, on each box converted outward to binary32, in binary32
+ * (precision 32): for the enclosure checks of tests against point evaluations
+ * in exact arithmetic (the reference's own tests evaluate its functions on
+ * random points and intervals, ivarp/test/test_util.hpp:113-235,
+ * test/cuda/cuda_test_fn.cuh:29-86).  bound_values[(i * n_bounds + k) * 3 + {0, 1,
+ * 2}] receives lb, ub and the possibly-undefined flag (0/1) of the expression of
+ * runtime bound k (the program's bound order); cons_values[(i * n_constraints +
+ * j) * 2 + {0, 1}] the IBool (definitely, possibly) of constraint j (the order of
+ * sqd_width_stats).  *n_bounds and *n_constraints receive the counts (NULL
+ * outputs: only those).  No counterpart in the reference. */
+int sqd_eval_program(sqd_core* core, int precision, const double* lb, const double* ub, size_t n,
+                     double* bound_values, uint8_t* cons_values, size_t* n_bounds, size_t* n_constraints);
 
 /* The number of initial subdivisions of the first dynamic variable (the range
  * sqd_prove_range selects from) and the number of static levels (the bits of
@@ -339,6 +375,8 @@ typedef struct sqd_kernel_stat {
     int32_t level;        /* static or dynamic level; -1 for other kernels */
     uint64_t launches;
     double milliseconds;  /* summed event time over the launches */
+    uint64_t children;    /* child cuboids those launches processed (static and split levels of
+                             proofs; 0 for other kernels) */
 } sqd_kernel_stat;
 
 /* enable != 0 turns profiling on and clears the statistics; 0 turns it off.

@@ -306,6 +306,14 @@ __device__ __forceinline__ void load_consts_f32(float* dst, const double* src) {
 }
 
 __device__ __forceinline__ unsigned long long input_base(const LevelArgs& a) { return a.in_base ? *a.in_base : 0ull; }
+// Inputs [base, base + n) of a level, n clamped to the input list's capacity (a
+// count past it was raised by a pass that could not store its survivors, which
+// set the overflow flag and fails the call).
+__device__ __forceinline__ unsigned long long input_count(const LevelArgs& a, unsigned long long base) {
+    const unsigned long long n = *a.in_count;
+    const unsigned long long room = a.in.cap > base ? a.in.cap - base : 0ull;
+    return n < room ? n : room;
+}
 
 template<class P>
 __device__ __forceinline__ void load_box(const BoxList& l, unsigned long long i, double (&lb)[P::N], double (&ub)[P::N]) {
@@ -465,7 +473,10 @@ __device__ __forceinline__ void static_decide(const LevelArgs& args, unsigned lo
             if(pos < args.out.cap) {
                 store_box<P>(args.out, pos, lb, ub);
                 if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
-            } else {
+            } else if(!LAST) {
+                // a level list is sized so that this cannot happen; the critical
+                // list's count keeps going past its capacity, which the host
+                // compares with it (and then repeats the final pass in parts)
                 atomicAdd(&args.counters->overflow, 1ull);
             }
         }
@@ -523,7 +534,10 @@ __device__ __forceinline__ void static_decide_hoisted(const LevelArgs& args, boo
             if(pos < args.out.cap) {
                 store_box<P>(args.out, pos, lb, ub);
                 if(args.out.origin) { args.out.origin[pos] = origin; args.out.leaf[pos] = leaf; }
-            } else {
+            } else if(!LAST) {
+                // a level list is sized so that this cannot happen; the critical
+                // list's count keeps going past its capacity, which the host
+                // compares with it (and then repeats the final pass in parts)
                 atomicAdd(&args.counters->overflow, 1ull);
             }
         }
@@ -554,7 +568,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(static_l
 void k_static_level(LevelArgs job0) {
     const LevelArgs args = job_args(job0);
     constexpr int SUBS = P::st_subs(L);
-    const unsigned long long n_items = *args.in_count;
+    const unsigned long long in_base = input_base(args);
+    const unsigned long long n_items = input_count(args, in_base);
     // the list holds at most undec_cap entries; a pass that found more counted
     // them (and raised the overflow flag, which fails the call) but stored only
     // those that fit
@@ -568,7 +583,6 @@ void k_static_level(LevelArgs job0) {
     const double* const s_c = nullptr;
     __shared__ PropScratch<P> s_prop;
     enter_round_down();
-    const unsigned long long in_base = input_base(args);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     unsigned long long c_cub = 0, c_leaf = 0, c_crit = 0;
     // every lane of a wave runs the same number of loop trips, so the wave-wide
@@ -668,8 +682,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     const float* const s_c32 = nullptr;
     __shared__ PropScratch<P> s_prop;
     enter_round_down();
-    const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
+    const unsigned long long n_items = input_count(args, in_base);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     const int cpl = group_size<SUBS, CPL>(args, n_items, stride);
     const int step = SUBS / cpl;                                                       // groups per parent
@@ -805,8 +819,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_group64(LevelA
     constexpr int CPL = children_per_lane<SUBS>();
     const double* const s_c = nullptr;   // constants are literals in the generated code (P::kc)
     enter_round_down();
-    const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
+    const unsigned long long n_items = input_count(args, in_base);
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     const int cpl = group_size<SUBS, CPL>(args, n_items, stride);
     const int step = SUBS / cpl;                                                       // groups per parent
@@ -866,8 +880,8 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
     const double* const s_c = nullptr;   // constants are literals in the generated code (P::kc)
     __shared__ PropScratch<P> s_prop;
     enter_round_down();
-    const unsigned long long n_items = *args.in_count;
     const unsigned long long in_base = input_base(args);
+    const unsigned long long n_items = input_count(args, in_base);
     // the initial queue may start from a range of the first variable's subdivisions
     constexpr bool RANGED = (MODE == INIT && J == 0);
     const unsigned long long r_lo = RANGED && args.init_range ? args.init_range[0] : 0ull;
@@ -1028,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_final_nostatic(BoxList fin, BoxList 
     fin = job_list(fin, off);
     out = job_list(out, off);
     block_counts = job_ptr(block_counts, off);
-    const unsigned long long n = *fin.count;
+    const unsigned long long n = *fin.count < fin.cap ? *fin.count : fin.cap;
     const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool v = i < n;
     double lb[P::N], ub[P::N];
@@ -1132,6 +1146,46 @@ __global__ __launch_bounds__(kBlock) void k_width_stats(const double* lb, const 
             atomicAdd(out + 3 * j + 1, static_cast<double>(cnt));
             atomicMax(reinterpret_cast<unsigned long long*>(out + 3 * j + 2), bits(mx));
         }
+    }
+}
+
+// Bound and constraint values on n boxes (SoA, stride n), for the enclosure
+// checks against point evaluations: the interval of bound k's expression on box
+// i goes to out_b[(i * NB + k) * 3 + {0: lb, 1: ub, 2: possibly undefined}], the
+// IBool of constraint j (cons_id order) to out_c[(i * NCONS + j) * 2 + {0:
+// definitely, 1: possibly}].  With Num = float the box is first converted
+// outward to binary32, as the prefilter converts its boxes, and the results are
+// widened exactly.
+template<class P, class Num>
+__global__ __launch_bounds__(kBlock) void k_eval_program(const double* lb, const double* ub, unsigned long long n,
+                                                         double* out_b, unsigned char* out_c) {
+    constexpr int N = P::N;
+    enter_round_down();
+    const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    Num bl[N], bu[N];
+#pragma unroll
+    for(int a = 0; a < N; ++a) {
+        const double l = lb[static_cast<unsigned long long>(a) * n + i];
+        const double u = ub[static_cast<unsigned long long>(a) * n + i];
+        if constexpr(std::is_same<Num, float>::value) { bl[a] = f32_down(l); bu[a] = f32_up(u); }
+        else { bl[a] = l; bu[a] = u; }
+    }
+    const Num* const C = nullptr;   // constants are literals in the generated code
+#pragma unroll 1
+    for(int k = 0; k < P::NB; ++k) {
+        const IVT<Num> v = P::bound_value(k, bl, bu, C);
+        double* o = out_b + (i * P::NB + static_cast<unsigned long long>(k)) * 3;
+        o[0] = static_cast<double>(v.lb);
+        o[1] = static_cast<double>(v.ub);
+        o[2] = v.und ? 1.0 : 0.0;
+    }
+#pragma unroll 1
+    for(int j = 0; j < P::NCONS; ++j) {
+        const IB c = P::cons_value(j, bl, bu, C);
+        unsigned char* o = out_c + (i * P::NCONS + static_cast<unsigned long long>(j)) * 2;
+        o[0] = c.def ? 1 : 0;
+        o[1] = c.poss ? 1 : 0;
     }
 }
 

@@ -48,6 +48,10 @@ struct KernelSet {
     int (*cons_id)(int j);
     hipError_t (*width_stats)(const double* lb, const double* ub, unsigned long long n, const double* consts,
                               double* out, hipStream_t s);
+    // Bound and constraint values on n boxes (k_eval_program), in binary64
+    // (precision 64) or binary32 (32).
+    hipError_t (*eval_program)(int precision, const double* lb, const double* ub, unsigned long long n,
+                               double* out_b, unsigned char* out_c, hipStream_t s);
 };
 
 void register_kernel_set(const KernelSet* ks);
@@ -203,6 +207,18 @@ hipError_t width_stats(const double* lb, const double* ub, unsigned long long n,
     return hipGetLastError();
 }
 
+template<class P>
+hipError_t eval_program(int precision, const double* lb, const double* ub, unsigned long long n, double* out_b,
+                        unsigned char* out_c, hipStream_t s) {
+    if(n == 0) return hipSuccess;
+    const unsigned grid = static_cast<unsigned>((n + dev::kBlock - 1) / dev::kBlock);
+    if(precision == 32)
+        hipLaunchKernelGGL((dev::k_eval_program<P, float>), dim3(grid), dim3(dev::kBlock), 0, s, lb, ub, n, out_b, out_c);
+    else
+        hipLaunchKernelGGL((dev::k_eval_program<P, double>), dim3(grid), dim3(dev::kBlock), 0, s, lb, ub, n, out_b, out_c);
+    return hipGetLastError();
+}
+
 inline hipError_t acos_hard(unsigned long long* out, hipStream_t s) {
     return hipMemcpyFromSymbolAsync(out, HIP_SYMBOL(dev::g_acos_hard), sizeof(unsigned long long), 0,
                                     hipMemcpyDeviceToHost, s);
@@ -231,6 +247,7 @@ KernelSet make_kernel_set(std::uint64_t fingerprint, const char* key, const doub
     k.NCONS = P::NCONS;
     k.cons_id = &detail::cons_id<P>;
     k.width_stats = &detail::width_stats<P>;
+    k.eval_program = &detail::eval_program<P>;
     return k;
 }
 

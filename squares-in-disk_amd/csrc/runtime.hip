@@ -72,6 +72,7 @@ struct DeviceState {
     unsigned long long f32_decided;           // static-level children pruned by the FP32 prefilter
     unsigned long long f32_checked;           // static-level children the FP32 prefilter evaluated
     unsigned long long undec_count[32];       // FP32 prefilter: undecided children of (mode, level), mode * 16 + level
+    unsigned long long part_n;                // inputs of one part of a final pass repeated in parts
 };
 
 static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a block of 64-bit words");
@@ -289,12 +290,18 @@ struct sqd_core {
     unsigned long long* d_gather = nullptr; // jobs control blocks side by side (k_gather_states)
     unsigned long long* d_slots = nullptr;
     unsigned long long n_slots = 0;        // >= the largest grid of a counting kernel
+    unsigned long long* d_discard = nullptr;   // count slots that are never folded (final passes repeated in parts)
+    unsigned long long* level_slots = nullptr; // the slots the level kernels add to: d_slots, or d_discard
     std::size_t state_bytes = 0;
     unsigned long long* d_hull = nullptr;
     unsigned clean = 0;               // jobs (from 0) whose hull records and count slots are known reset
     unsigned char* d_has = nullptr;
     sqd::DevList stack, lvl[2], fin, spl, crit, scratch;
     std::uint64_t device_bytes = 0;  // device memory allocated by sqd_core_create
+    sqd_critical_fn crit_fn = nullptr;       // critical sink (sqd_core_set_critical_sink)
+    void* crit_user = nullptr;
+    std::vector<double> crit_host;           // one part of the criticals on its way to the sink
+    unsigned long long crit_delivered = 0;   // criticals handed to the sink by the current call
     sqd_progress_fn progress_fn = nullptr;   // progress reports of sqd_prove
     void* progress_user = nullptr;
     double progress_start = 0, progress_every = 0;
@@ -309,6 +316,8 @@ struct sqd_core {
     unsigned long long k_max = 0;    // dynamic cuboids per round
     unsigned long long split_fanout = 1;
     std::vector<std::uint32_t> st_subs, dyn_subs, dyn_initial;
+    std::vector<double> box_init;                  // host copy of the compile-time box (uploaded per job)
+    unsigned long long range_init[2] = {0, 1};     // host copy of the full initial range
     // profiling: events around each launch, resolved at the next stream sync
     struct Pending { int kind; hipEvent_t a, b; };
     bool profiling = false;
@@ -433,9 +442,43 @@ const DeviceState& finish_state(sqd_core* c) {
             throw Failure(SQD_ERR_CAPACITY, "a device level buffer overflowed; raise sqd_settings.level_bytes");
         if(c->h_state[j].frontier_full)
             throw Failure(SQD_ERR_CAPACITY, "frontier capacity exhausted; raise sqd_settings.frontier_capacity");
+        if(c->h_state[j].crit_count > c->crit.cap && !(c->crit_fn && c->ny == 1))
+            throw Failure(SQD_ERR_CAPACITY, "more critical cuboids (" + std::to_string(c->h_state[j].crit_count) +
+                                                ") than sqd_settings.critical_capacity (" + std::to_string(c->crit.cap) +
+                                                "); set a critical sink (sqd_core_set_critical_sink) to receive every "
+                                                "one, or raise the capacity");
     }
     return *c->h_state;
 }
+// With profiling on: adds the child cuboids of the round whose state was copied
+// last to the statistics of the level kernels that timed them.  A static level's
+// children are its inputs times its subdivisions; the inputs of level 0 are the
+// round's popped cuboids (non-final) or its final list, those of level L the
+// survivors of level L - 1; the split levels alike from the split list.
+void count_children(sqd_core* c) {
+    if(!c->profiling) return;
+    auto add = [&](int kind, unsigned long long n) {
+        auto it = c->stats.find(kind);
+        if(it != c->stats.end()) it->second.children += n;
+    };
+    const int S = c->ks->NSTAT, D = c->ks->NDYN;
+    for(unsigned j = 0; j < c->ny; ++j) {
+        const DeviceState& h = c->h_state[j];
+        for(int mode : {dev::NONFINAL, dev::FINAL}) {
+            unsigned long long in = mode == dev::NONFINAL ? h.round_k : h.fin_count;
+            for(int L = 0; L < S; ++L) {
+                add(mode * 16 + L, in * c->st_subs[static_cast<std::size_t>(L)]);
+                in = h.level_count[mode * 16 + L];
+            }
+        }
+        unsigned long long in = h.spl_count;
+        for(int J = 0; J < D; ++J) {
+            add(dev::SPLIT * 16 + J, in * c->dyn_subs[static_cast<std::size_t>(J)]);
+            in = h.level_count[dev::SPLIT * 16 + J];
+        }
+    }
+}
+
 // Whether some job of the last state copy still has cuboids on its frontier.
 bool frontier_left(const sqd_core* c) {
     for(unsigned j = 0; j < c->ny; ++j)
@@ -465,7 +508,7 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     a.in_count = count;
     a.hull = c->d_hull;
     a.counters = &c->d_state->counters;
-    a.block_counts = c->d_slots;
+    a.block_counts = c->level_slots;
     a.consts = c->d_consts;
     a.generations = c->settings.generation_count;
     a.undec = c->d_undec;
@@ -478,32 +521,44 @@ dev::LevelArgs level_args(sqd_core* c, const dev::BoxList& in, const unsigned lo
     return a;
 }
 
+// Static level L of `mode` over the inputs [*base, *base + *count) of `in`
+// (base may be null: 0), at most `max_items` of them, writing `out`.  Its
+// undecided-list counter undec_count[mode * 16 + L] must be zero on entry.
+void run_static_level(sqd_core* c, int mode, int L, const dev::BoxList& in, const unsigned long long* base,
+                      const unsigned long long* count, unsigned long long max_items, const dev::BoxList& out) {
+    dev::BoxList src = in;
+    if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
+    dev::LevelArgs a = level_args(c, src, base, count, out);
+    a.undec_count = &c->d_state->undec_count[mode * 16 + L];
+    const unsigned long long children = max_items * c->st_subs[static_cast<std::size_t>(L)];
+    launch(c, mode * 16 + L, "static level launch",
+           [&] {
+               const bool f32 = (c->settings.fp32_prefilter >> L) & 1u;
+               const bool grp = (c->settings.group_levels >> L) & 1u;
+               const int m = mode | (f32 ? dev::kModeF32 : grp ? dev::kModeGroup : 0);
+               return c->ks->static_level(m, L, a, children, c->cus, c->grid_override, c->stream);
+           });
+}
+
 // Static levels over the inputs [*base, *base + *count) of `in` (base may be
 // null: 0), at most `max_items` of them.  NONFINAL joins the leaves into d_hull;
 // FINAL appends critical leaves to `crit_out`.  Level L writes lvl[L % 2] with
-// the counter level_count[mode * 16 + L], which must be zero on entry.
+// the counter level_count[mode * 16 + L], which must be zero on entry.  With
+// `levels` >= 0 only the first `levels` levels run.
 void run_static(sqd_core* c, int mode, const dev::BoxList& in, const unsigned long long* base,
-                const unsigned long long* count, unsigned long long max_items, const dev::BoxList& crit_out) {
+                const unsigned long long* count, unsigned long long max_items, const dev::BoxList& crit_out,
+                int levels = -1) {
     const int S = c->ks->NSTAT;
+    const int upto = levels < 0 ? S : std::min(levels, S);
     dev::BoxList prev = in;
     const unsigned long long* prev_base = base;
     const unsigned long long* prev_count = count;
     unsigned long long items = max_items;
-    for(int L = 0; L < S; ++L) {
+    for(int L = 0; L < upto; ++L) {
         const bool last = (L == S - 1);
         dev::BoxList out = last ? crit_out : c->lvl[L % 2].view(&c->d_state->level_count[mode * 16 + L]);
-        dev::BoxList src = prev;
-        if(L == 0) { src.origin = nullptr; src.leaf = nullptr; }  // the input index is the origin
-        dev::LevelArgs a = level_args(c, src, prev_base, prev_count, out);
-        a.undec_count = &c->d_state->undec_count[mode * 16 + L];
+        run_static_level(c, mode, L, prev, prev_base, prev_count, items, out);
         const unsigned long long children = items * c->st_subs[static_cast<std::size_t>(L)];
-        launch(c, mode * 16 + L, "static level launch",
-               [&] {
-                   const bool f32 = (c->settings.fp32_prefilter >> L) & 1u;
-                   const bool grp = (c->settings.group_levels >> L) & 1u;
-                   const int m = mode | (f32 ? dev::kModeF32 : grp ? dev::kModeGroup : 0);
-                   return c->ks->static_level(m, L, a, children, c->cus, c->grid_override, c->stream);
-               });
         prev = out;
         prev_base = nullptr;
         prev_count = out.count;
@@ -703,6 +758,94 @@ void enqueue_round(sqd_core* c) {
     run_dynamic(c, dev::SPLIT, c->spl.view(), c->spl.count, K, c->stack.view());
 }
 
+// Hands the criticals [0, k) of the critical list (k <= its capacity) to the sink.
+void deliver_criticals(sqd_core* c, unsigned long long k) {
+    if(k == 0) return;
+    c->crit_host.assign(2 * static_cast<std::size_t>(c->N) * k, 0.0);
+    double* lb = c->crit_host.data();
+    double* ub = lb + static_cast<std::size_t>(c->N) * k;
+    download_soa(c, c->crit, k, lb, ub, k);
+    c->crit_delivered += k;
+    c->crit_fn(c->crit_user, lb, ub, static_cast<size_t>(k), static_cast<size_t>(k));
+}
+
+// Entries [off, cap) of a list as a list of their own, counted by `count`.
+dev::BoxList list_from(const dev::BoxList& l, unsigned long long off, unsigned long long* count) {
+    dev::BoxList v = l;
+    v.lb = l.lb + off;
+    v.ub = l.ub + off;
+    if(l.depth) v.depth = l.depth + off;
+    if(l.origin) v.origin = l.origin + off;
+    if(l.leaf) v.leaf = l.leaf + off;
+    v.cap = l.cap > off ? l.cap - off : 0;
+    v.count = count;
+    return v;
+}
+
+// Hands the k criticals of the last final pass (over the n_fin boxes of the fin
+// list, job 0) to the sink, then empties the critical list.  When the pass found
+// more criticals than the list holds, the pass is repeated in parts whose
+// criticals fit: the levels before the last over the whole fin list, then the
+// last level over slices of its inputs (an input yields at most that level's
+// subdivisions in criticals), with the count slots of the repetition discarded so
+// that nothing is counted twice.  The repetition runs the same kernels on the
+// same boxes, so its criticals are the ones the pass counted.
+void drain_criticals(sqd_core* c, unsigned long long k, unsigned long long n_fin) {
+    DeviceState* st = c->d_state;
+    if(k <= c->crit.cap) {
+        deliver_criticals(c, k);
+    } else {
+        struct Discard {
+            sqd_core* c;
+            explicit Discard(sqd_core* core) : c(core) { c->level_slots = c->d_discard; }
+            ~Discard() { c->level_slots = c->d_slots; }
+        } discard(c);
+        const int S = c->ks->NSTAT;
+        const unsigned long long cap = c->crit.cap;
+        dev::BoxList src = c->fin.view();
+        unsigned long long n_in = std::min<unsigned long long>(n_fin, c->fin.cap);
+        if(S >= 2) {
+            levels_reset(c);
+            run_static(c, dev::FINAL, c->fin.view(), nullptr, c->fin.count, n_in, c->crit.view(), S - 1);
+            n_in = read_state(c).level_count[dev::FINAL * 16 + S - 2];
+            src = c->lvl[(S - 2) % 2].view();
+        }
+        n_in = std::min(n_in, src.cap);
+        // a part's children must fit the critical list and, on a prefiltered or
+        // grouped last level, the undecided list; without static levels a part
+        // is at most a round's final list
+        const unsigned long long subs = S > 0 ? c->st_subs[static_cast<std::size_t>(S - 1)] : 1ull;
+        const unsigned long long room = c->undec_cap ? std::min(cap, c->undec_cap) : cap;
+        unsigned long long per = std::max<unsigned long long>(1, room / subs);
+        if(S == 0) per = std::min(per, c->k_max);
+        for(unsigned long long off = 0; off < n_in; off += per) {
+            const unsigned long long m = std::min(per, n_in - off);
+            set_count(c, &st->part_n, m);
+            set_count(c, c->crit.count, 0);
+            const dev::BoxList part = list_from(src, off, &st->part_n);
+            if(S == 0) {
+                launch(c, KIND_FINAL_NOSTATIC, "final", [&] {
+                    return c->ks->final_nostatic(part, c->crit.view(), c->d_discard,
+                                                 static_cast<unsigned>((m + 255) / 256), 1u, 0ull, c->stream);
+                });
+            } else {
+                set_count(c, &st->undec_count[dev::FINAL * 16 + S - 1], 0);
+                run_static_level(c, dev::FINAL, S - 1, part, nullptr, &st->part_n, m, c->crit.view());
+            }
+            deliver_criticals(c, std::min(read_state(c).crit_count, cap));
+        }
+    }
+    set_count(c, c->crit.count, 0);
+}
+
+// After a round of a proof on a core of one job: hands the round's criticals to
+// the sink, if one is set.
+void drain_round(sqd_core* c) {
+    if(!c->crit_fn || c->ny != 1) return;
+    const DeviceState& h = c->h_state[0];
+    if(h.crit_count) drain_criticals(c, h.crit_count, h.fin_count);
+}
+
 // Proves the initial ranges [first[j], last[j]) of n jobs, leaving each job's
 // final state in h_state[j].  A core of one job runs one grid row; a batch core
 // always runs all its rows (the jobs beyond n get empty ranges and do no work),
@@ -730,6 +873,7 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
     } rows(c);
     set_ranges(c, r);
     begin_call(c);
+    c->crit_delivered = 0;
     const bool graphs = c->settings.graphs && !c->profiling && c->proofs > 0;
     if(graphs && !c->g_first) {
         c->g_first = capture_graph(c, [&] { enqueue_initial_queue(c); enqueue_round(c); enqueue_state_copy(c); });
@@ -750,11 +894,13 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
     if(graphs) {
         hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
         finish_state(c);
+        drain_round(c);
         ++rounds;
         while(frontier_left(c)) {
             progress();
             hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
             finish_state(c);
+            drain_round(c);
             ++rounds;
         }
     } else {
@@ -763,14 +909,150 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
             if(rounds > 0) progress();
             enqueue_round(c);
             read_state(c);
+            count_children(c);
+            drain_round(c);
             ++rounds;
         } while(frontier_left(c));
     }
+    if(c->crit_fn && c->ny == 1 && c->crit_delivered != c->h_state[0].counters.criticals)
+        throw Failure(SQD_ERR_INTERNAL, "criticals handed to the sink differ from the criticals counted");
     ++c->proofs;
     end_call(c);
     if(progress_reports) report_progress(c, rounds, true, SQD_OK);
     auto t1 = std::chrono::steady_clock::now();
     if(seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Settings, launch shape and the layout of one job's device memory for a core
+// of `program` on `device` (nothing is allocated): the sizes follow from the
+// settings, and the rounds are bounded so that no level list and no undecided
+// list can overflow.  `state_words` receives the place of the control block in
+// the layout.
+void configure_core(sqd_core* c, int device, const sqd_program* program, const sqd_settings* settings, Layout& lay,
+                    unsigned long long*& state_words) {
+    if(!program->ks)
+        throw Failure(SQD_ERR_NO_KERNELS, "no kernels are compiled for this program (fingerprint not registered)");
+    c->device = device;
+    c->program = program;
+    c->ks = program->ks;
+    c->N = program->prog.nargs;
+    if(c->N < 1 || c->N > 64) throw std::invalid_argument("programs need 1 to 64 variables");
+    if(settings) c->settings = *settings;
+    else sqd_settings_default(&c->settings);
+    sqd_settings& s = c->settings;
+    if(s.frontier_capacity == 0) s.frontier_capacity = 1ull << 23;
+    if(s.level_bytes == 0) s.level_bytes = 1ull << 30;
+    if(s.critical_capacity == 0) s.critical_capacity = 1ull << 20;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    c->cus = static_cast<unsigned>(prop.multiProcessorCount);
+    c->grid_override = s.grid_blocks;
+    const auto& prog = program->prog;
+    for(const auto& sp : prog.splits) {
+        if(sp.dynamic) { c->dyn_subs.push_back(sp.subs); c->dyn_initial.push_back(sp.initial); }
+        else c->st_subs.push_back(sp.subs);
+    }
+    unsigned used = 0;   // the levels the masks can select
+    for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L) used |= 1u << L;
+    c->fp32 = (s.fp32_prefilter & used) != 0;
+
+    // level buffers hold the survivors of one level; size K for the worst case
+    const unsigned long long per_entry = 16ull * static_cast<unsigned long long>(c->N) + 12ull;
+    const unsigned long long lvl_cap = std::max<unsigned long long>(1024, s.level_bytes / per_entry);
+    unsigned long long worst_static = 1, worst_split = 1;
+    for(std::size_t L = 0; L + 1 < c->st_subs.size(); ++L) worst_static = std::max<unsigned long long>(worst_static, product(c->st_subs, L + 1));
+    for(std::size_t J = 0; J + 1 < c->dyn_subs.size(); ++J) worst_split = std::max<unsigned long long>(worst_split, product(c->dyn_subs, J + 1));
+    unsigned long long worst_init = product(c->dyn_initial, c->dyn_initial.size() > 0 ? c->dyn_initial.size() - 1 : 0);
+    if(worst_init > lvl_cap) throw Failure(SQD_ERR_CAPACITY, "initial queue levels do not fit the level buffers");
+    c->split_fanout = product(c->dyn_subs, c->dyn_subs.size());
+    unsigned long long kmax = s.dequeue_buffer_size ? s.dequeue_buffer_size : 8192;
+    kmax = std::min(kmax, lvl_cap / worst_static);
+    kmax = std::min(kmax, lvl_cap / worst_split);
+    // the FP32 prefilter and the binary64 group test list the children they
+    // leave undecided (8 bytes each, level_bytes of them), up to all children
+    // of the level, so a round is bounded by the largest such level's children
+    // per dynamic cuboid
+    const unsigned long long undec_cap = std::max<unsigned long long>(1024, s.level_bytes / 8);
+    unsigned long long worst_undec = 1;
+    for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L)
+        if(((s.group_levels | s.fp32_prefilter) >> L) & 1u)
+            worst_undec = std::max<unsigned long long>(worst_undec, product(c->st_subs, L + 1));
+    kmax = std::min(kmax, undec_cap / worst_undec);
+    if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
+    c->k_max = kmax;
+    // a critical sink receives a final pass in parts of at least one input of
+    // the last static level, so the critical list holds at least its children
+    if(!c->st_subs.empty()) s.critical_capacity = std::max<unsigned long long>(s.critical_capacity, c->st_subs.back());
+    // count slots: one per block of the largest counting grid (level kernels
+    // run at most 8 blocks of 256 threads per CU; the small per-round
+    // kernels need at most k_max / 256 + 1 blocks)
+    c->n_slots = std::max<unsigned long long>(8ull * c->cus, kmax / 256 + 2);
+    if(c->grid_override > c->n_slots) c->grid_override = static_cast<unsigned>(c->n_slots);
+    c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * dev::kSlotWords * sizeof(unsigned long long);
+    // one job's memory, laid out once; a batch holds `jobs` copies of it
+    lay.take(state_words, c->state_bytes / sizeof(unsigned long long));
+    lay.take(c->d_discard, static_cast<std::size_t>(c->n_slots) * dev::kSlotWords);
+    const unsigned long long stack_cap = std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1);
+    c->stack.lay(lay, c->N, stack_cap, true, false);
+    for(auto& l : c->lvl) l.lay(lay, c->N, lvl_cap, true, true);
+    c->fin.lay(lay, c->N, kmax, true, false);
+    c->spl.lay(lay, c->N, kmax, true, false);
+    c->crit.lay(lay, c->N, s.critical_capacity, false, false);
+    c->scratch.lay(lay, c->N, std::max<unsigned long long>(kmax, 1024), true, false);
+    lay.take(c->d_hull, static_cast<std::size_t>(kmax) * c->N * 4);
+    lay.take(c->d_range, 2);
+    if(c->fp32 || (s.group_levels & used)) {
+        c->undec_cap = undec_cap;
+        lay.take(c->d_undec, static_cast<std::size_t>(undec_cap));
+    }
+    lay.take(c->d_has, static_cast<std::size_t>(kmax));
+    const std::size_t n_consts = 2 * static_cast<std::size_t>(std::max(1, c->ks->NC));
+    lay.take(c->d_consts, n_consts);
+    lay.take(c->d_box_ct, 2 * static_cast<std::size_t>(c->N));
+    lay.take(c->d_box_init, 2 * static_cast<std::size_t>(c->N));
+    c->jobs = s.jobs ? s.jobs : 1u;
+    if(c->jobs > kMaxJobs) throw std::invalid_argument("sqd_settings.jobs: at most 64 jobs per batch");
+    c->job_stride = lay.stride();
+    // the sizing rules the level kernels rely on (they also clamp their input
+    // counts to the list capacities)
+    if(kmax > c->fin.cap || kmax > c->spl.cap || kmax * worst_static > lvl_cap || kmax * worst_split > lvl_cap ||
+       (c->undec_cap && kmax * worst_undec > c->undec_cap))
+        throw Failure(SQD_ERR_INTERNAL, "core sizing rule violated");
+}
+
+// Device pointers into the control block once the job memory is placed.
+void bind_state(sqd_core* c, unsigned long long* state_words) {
+    c->d_state = reinterpret_cast<DeviceState*>(state_words);
+    c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
+    c->level_slots = c->d_slots;
+    DeviceState* st = c->d_state;
+    c->stack.count = &st->stack_count;
+    c->fin.count = &st->fin_count;
+    c->spl.count = &st->spl_count;
+    c->crit.count = &st->crit_count;
+    c->scratch.count = &st->scratch_count;
+}
+
+// Every job's copy of the constants, the compile-time box and the full range
+// (enqueued on the core's stream from host memory).
+void upload_job_constants(sqd_core* c) {
+    const auto& prog = c->program->prog;
+    c->range_init[0] = 0ull;
+    c->range_init[1] = c->dyn_initial.empty() ? 1ull : c->dyn_initial[0];
+    c->box_init.assign(prog.box_lb.begin(), prog.box_lb.end());
+    c->box_init.insert(c->box_init.end(), prog.box_ub.begin(), prog.box_ub.end());
+    const std::size_t n_consts = 2 * static_cast<std::size_t>(std::max(1, c->ks->NC));
+    for(unsigned j = 0; j < c->jobs; ++j) {
+        const std::size_t off = static_cast<std::size_t>(c->job_stride) * j;
+        auto at = [&](auto* q) { return reinterpret_cast<char*>(q) + off; };
+        hip_check(hipMemcpyAsync(at(c->d_range), c->range_init, sizeof c->range_init, hipMemcpyHostToDevice, c->stream),
+                  "initial range");
+        hip_check(hipMemcpyAsync(at(c->d_consts), c->ks->consts, sizeof(double) * n_consts, hipMemcpyHostToDevice,
+                                 c->stream), "upload constants");
+        hip_check(hipMemcpyAsync(at(c->d_box_ct), c->box_init.data(), c->box_init.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream), "upload box");
+    }
 }
 
 } // namespace
@@ -852,8 +1134,6 @@ int sqd_program_has_kernels(const sqd_program* p) { return (p && p->ks) ? 1 : 0;
 int sqd_core_create(int device, const sqd_program* program, const sqd_settings* settings, sqd_core** out) {
     return guarded([&] {
         if(!program || !out) throw std::invalid_argument("null argument");
-        if(!program->ks)
-            throw Failure(SQD_ERR_NO_KERNELS, "no kernels are compiled for this program (fingerprint not registered)");
         std::unique_lock<std::mutex> alloc_lock(g_capture_mutex);
         struct Destroy {
             std::unique_lock<std::mutex>* lock;
@@ -861,114 +1141,19 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         };
         std::unique_ptr<sqd_core, Destroy> c(new sqd_core, Destroy{&alloc_lock});   // frees what was allocated if creation fails
         const std::uint64_t bytes0 = t_dalloc_bytes;
-        c->device = device;
-        c->program = program;
-        c->ks = program->ks;
-        c->N = program->prog.nargs;
-        if(c->N < 1 || c->N > 64) throw std::invalid_argument("programs need 1 to 64 variables");
-        if(settings) c->settings = *settings;
-        else sqd_settings_default(&c->settings);
-        sqd_settings& s = c->settings;
-        if(s.frontier_capacity == 0) s.frontier_capacity = 1ull << 23;
-        if(s.level_bytes == 0) s.level_bytes = 1ull << 30;
-        if(s.critical_capacity == 0) s.critical_capacity = 1ull << 20;
-        hip_check(hipSetDevice(device), "hipSetDevice");
-        hipDeviceProp_t prop;
-        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        c->cus = static_cast<unsigned>(prop.multiProcessorCount);
-        c->grid_override = s.grid_blocks;
-        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-        const auto& prog = program->prog;
-        for(const auto& sp : prog.splits) {
-            if(sp.dynamic) { c->dyn_subs.push_back(sp.subs); c->dyn_initial.push_back(sp.initial); }
-            else c->st_subs.push_back(sp.subs);
-        }
-        unsigned used = 0;   // the levels the mask can select
-        for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L) used |= 1u << L;
-        c->fp32 = (s.fp32_prefilter & used) != 0;
-
-        // level buffers hold the survivors of one level; size K for the worst case
-        const unsigned long long per_entry = 16ull * static_cast<unsigned long long>(c->N) + 12ull;
-        const unsigned long long lvl_cap = std::max<unsigned long long>(1024, s.level_bytes / per_entry);
-        unsigned long long worst_static = 1, worst_split = 1;
-        for(std::size_t L = 0; L + 1 < c->st_subs.size(); ++L) worst_static = std::max<unsigned long long>(worst_static, product(c->st_subs, L + 1));
-        for(std::size_t J = 0; J + 1 < c->dyn_subs.size(); ++J) worst_split = std::max<unsigned long long>(worst_split, product(c->dyn_subs, J + 1));
-        unsigned long long worst_init = product(c->dyn_initial, c->dyn_initial.size() > 0 ? c->dyn_initial.size() - 1 : 0);
-        if(worst_init > lvl_cap) throw Failure(SQD_ERR_CAPACITY, "initial queue levels do not fit the level buffers");
-        c->split_fanout = product(c->dyn_subs, c->dyn_subs.size());
-        unsigned long long kmax = s.dequeue_buffer_size ? s.dequeue_buffer_size : 8192;
-        kmax = std::min(kmax, lvl_cap / worst_static);
-        kmax = std::min(kmax, lvl_cap / worst_split);
-        // binary64 group test: the children of the groups it leaves go to the
-        // undecided list (as many entries as a level buffer holds boxes), up to
-        // all children of the level, so a round is bounded by the largest such
-        // level's children per dynamic cuboid
-        unsigned long long worst_group = 1;
-        for(std::size_t L = 0; L < c->st_subs.size() && L < 32; ++L)
-            if(((s.group_levels & ~s.fp32_prefilter) >> L) & 1u)
-                worst_group = std::max<unsigned long long>(worst_group, product(c->st_subs, L + 1));
-        kmax = std::min(kmax, lvl_cap / worst_group);
-        if(kmax == 0) throw Failure(SQD_ERR_CAPACITY, "level buffers too small for one cuboid");
-        c->k_max = kmax;
-        // count slots: one per block of the largest counting grid (level kernels
-        // run at most 8 blocks of 256 threads per CU; the small per-round
-        // kernels need at most k_max / 256 + 1 blocks)
-        c->n_slots = std::max<unsigned long long>(8ull * c->cus, kmax / 256 + 2);
-        if(c->grid_override > c->n_slots) c->grid_override = static_cast<unsigned>(c->n_slots);
-        c->state_bytes = sizeof(DeviceState) + static_cast<std::size_t>(c->n_slots) * dev::kSlotWords * sizeof(unsigned long long);
-        // one job's memory, laid out once; a batch holds `jobs` copies of it
         Layout lay;
         unsigned long long* state_words = nullptr;
-        lay.take(state_words, c->state_bytes / sizeof(unsigned long long));
-        const unsigned long long stack_cap = std::max<unsigned long long>(s.frontier_capacity, prog.initial_queue_size + 1);
-        c->stack.lay(lay, c->N, stack_cap, true, false);
-        for(auto& l : c->lvl) l.lay(lay, c->N, lvl_cap, true, true);
-        c->fin.lay(lay, c->N, kmax, true, false);
-        c->spl.lay(lay, c->N, kmax, true, false);
-        c->crit.lay(lay, c->N, s.critical_capacity, false, false);
-        c->scratch.lay(lay, c->N, std::max<unsigned long long>(kmax, 1024), true, false);
-        lay.take(c->d_hull, static_cast<std::size_t>(kmax) * c->N * 4);
-        lay.take(c->d_range, 2);
-        if(c->fp32 || (s.group_levels & used)) {   // as many undecided children as a level buffer holds boxes
-            c->undec_cap = lvl_cap;
-            lay.take(c->d_undec, static_cast<std::size_t>(lvl_cap));
-        }
-        lay.take(c->d_has, static_cast<std::size_t>(kmax));
-        const std::size_t n_consts = 2 * static_cast<std::size_t>(std::max(1, c->ks->NC));
-        lay.take(c->d_consts, n_consts);
-        lay.take(c->d_box_ct, 2 * static_cast<std::size_t>(c->N));
-        lay.take(c->d_box_init, 2 * static_cast<std::size_t>(c->N));
-        c->jobs = s.jobs ? s.jobs : 1u;
-        if(c->jobs > kMaxJobs) throw std::invalid_argument("sqd_settings.jobs: at most 64 jobs per batch");
-        c->job_stride = lay.stride();
+        configure_core(c.get(), device, program, settings, lay, state_words);
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         c->d_jobs = dalloc<char>(static_cast<std::size_t>(c->job_stride) * c->jobs);
         lay.place(c->d_jobs);
-        c->d_state = reinterpret_cast<DeviceState*>(state_words);
-        c->d_slots = reinterpret_cast<unsigned long long*>(c->d_state + 1);
-        DeviceState* st = c->d_state;
-        c->stack.count = &st->stack_count;
-        c->fin.count = &st->fin_count;
-        c->spl.count = &st->spl_count;
-        c->crit.count = &st->crit_count;
-        c->scratch.count = &st->scratch_count;
+        bind_state(c.get(), state_words);
         if(c->jobs > 1) c->d_gather = dalloc<unsigned long long>(sizeof(DeviceState) / sizeof(unsigned long long) * c->jobs);
         void* hs = nullptr;
         hip_check(hipHostMalloc(&hs, sizeof(DeviceState) * c->jobs, hipHostMallocDefault), "hipHostMalloc");
         c->h_state = static_cast<DeviceState*>(hs);
         std::memset(c->h_state, 0, sizeof(DeviceState) * c->jobs);
-        // every job's copy of the constants, the compile-time box and the full range
-        const unsigned long long full[2] = {0ull, c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]};
-        std::vector<double> box(prog.box_lb);
-        box.insert(box.end(), prog.box_ub.begin(), prog.box_ub.end());
-        for(unsigned j = 0; j < c->jobs; ++j) {
-            const std::size_t off = static_cast<std::size_t>(c->job_stride) * j;
-            auto at = [&](auto* q) { return reinterpret_cast<char*>(q) + off; };
-            hip_check(hipMemcpyAsync(at(c->d_range), full, sizeof full, hipMemcpyHostToDevice, c->stream), "initial range");
-            hip_check(hipMemcpyAsync(at(c->d_consts), c->ks->consts, sizeof(double) * n_consts, hipMemcpyHostToDevice,
-                                     c->stream), "upload constants");
-            hip_check(hipMemcpyAsync(at(c->d_box_ct), box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice,
-                                     c->stream), "upload box");
-        }
+        upload_job_constants(c.get());
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->device_bytes = t_dalloc_bytes - bytes0;
         *out = c.release();
@@ -1074,12 +1259,20 @@ int sqd_final(sqd_core* c, const double* lb, const double* ub, size_t n,
             return c->ks->final_nostatic(c->fin.view(), c->crit.view(), c->d_slots,
                                          static_cast<unsigned>((n + 255) / 256 + 1), 1u, 0ull, c->stream);
         });
+        c->crit_delivered = 0;
         const DeviceState& h = read_state(c);
         const unsigned long long k = h.crit_count;
         const sqd_counts r = counts_of(h);
+        if(c->crit_fn) {                  // every critical to the sink, none to the output arrays
+            drain_criticals(c, k, n);
+            sync(c);
+            if(c->crit_delivered != r.num_critical_cuboids)
+                throw Failure(SQD_ERR_INTERNAL, "criticals handed to the sink differ from the criticals counted");
+        }
         end_call(c);
-        if(n_critical) *n_critical = k;
-        if(crit_lb && crit_ub) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+        if(n_critical) *n_critical = c->crit_fn ? r.num_critical_cuboids : k;
+        if(crit_lb && crit_ub && !c->crit_fn)
+            download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
         if(counts) *counts = r;
     });
 }
@@ -1117,10 +1310,12 @@ int sqd_prove_range(sqd_core* c, uint32_t first, uint32_t last, sqd_counts* coun
         if(!c) throw std::invalid_argument("null core");
         prove_jobs(c, 1, &first, &last, seconds, true);
         const DeviceState& h = c->h_state[0];
-        const unsigned long long k = h.crit_count;
+        const bool sink = c->crit_fn && c->jobs == 1;   // the criticals went to the sink
+        const unsigned long long k = sink ? h.counters.criticals : h.crit_count;
         if(counts) *counts = counts_of(h);
         if(n_critical) *n_critical = k;
-        if(crit_lb && crit_ub && k) download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
+        if(crit_lb && crit_ub && k && !sink)
+            download_soa(c, c->crit, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
     });
     if(rc != SQD_OK && c && c->progress_fn) {     // the reference's observe_error
         const std::string msg = t_error;
@@ -1161,6 +1356,14 @@ int sqd_core_job_criticals(sqd_core* c, size_t job, double* crit_lb, double* cri
             l.ub = reinterpret_cast<double*>(reinterpret_cast<char*>(l.ub) + off);
             download_soa(c, l, std::min<unsigned long long>({k, capacity, c->crit.cap}), crit_lb, crit_ub, capacity);
         }
+    });
+}
+
+int sqd_core_set_critical_sink(sqd_core* c, sqd_critical_fn fn, void* user) {
+    return guarded([&] {
+        if(!c) throw std::invalid_argument("null core");
+        c->crit_fn = fn;
+        c->crit_user = fn ? user : nullptr;
     });
 }
 
@@ -1262,6 +1465,46 @@ int sqd_width_stats(sqd_core* c, const double* lb, const double* ub, size_t n, s
             std::memcpy(&mx, &mb, sizeof mx);
             w.max_ratio = mx;
         }
+    });
+}
+
+int sqd_eval_program(sqd_core* c, int precision, const double* lb, const double* ub, size_t n, double* bound_values,
+                     uint8_t* cons_values, size_t* n_bounds, size_t* n_constraints) {
+    return guarded([&] {
+        if(!c || !n_bounds || !n_constraints || (n && (!lb || !ub))) throw std::invalid_argument("null argument");
+        if(precision != 64 && precision != 32) throw std::invalid_argument("precision must be 64 or 32");
+        const std::size_t nb = static_cast<std::size_t>(c->ks->NB), nc = static_cast<std::size_t>(c->ks->NCONS);
+        *n_bounds = nb;
+        *n_constraints = nc;
+        if(!bound_values || !cons_values || n == 0) return;
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        sync(c);
+        const std::size_t box_bytes = sizeof(double) * static_cast<std::size_t>(c->N) * n;
+        double *d_lb = nullptr, *d_ub = nullptr, *d_b = nullptr;
+        unsigned char* d_c = nullptr;
+        std::unique_lock<std::mutex> lock(g_capture_mutex);
+        auto release = [&] {
+            for(void* q : {static_cast<void*>(d_lb), static_cast<void*>(d_ub), static_cast<void*>(d_b),
+                           static_cast<void*>(d_c)})
+                if(q) (void)hipFree(q);
+        };
+        try {
+            d_lb = dalloc<double>(static_cast<std::size_t>(c->N) * n);
+            d_ub = dalloc<double>(static_cast<std::size_t>(c->N) * n);
+            d_b = dalloc<double>(std::max<std::size_t>(1, n * nb * 3));
+            d_c = dalloc<unsigned char>(std::max<std::size_t>(1, n * nc * 2));
+            hip_check(hipMemcpyAsync(d_lb, lb, box_bytes, hipMemcpyHostToDevice, c->stream), "upload boxes");
+            hip_check(hipMemcpyAsync(d_ub, ub, box_bytes, hipMemcpyHostToDevice, c->stream), "upload boxes");
+            hip_check(c->ks->eval_program(precision, d_lb, d_ub, n, d_b, d_c, c->stream), "evaluation launch");
+            if(nb)
+                hip_check(hipMemcpyAsync(bound_values, d_b, n * nb * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                         c->stream), "download bound values");
+            if(nc)
+                hip_check(hipMemcpyAsync(cons_values, d_c, n * nc * 2, hipMemcpyDeviceToHost, c->stream),
+                          "download constraint values");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        } catch(...) { release(); throw; }
+        release();
     });
 }
 

@@ -318,6 +318,20 @@ std::string generate(const flat::Program& p, const std::string& key) {
     body << "        switch(j) {\n";
     for(std::size_t j = 0; j < cons.size(); ++j) body << "        case " << j << ": return d" << cons[j] << "(lb, ub, C);\n";
     body << "        default: return IVT<Num>{Num(1), Num(0), true};\n        }\n    }\n";
+    // values for the enclosure checks: the interval of bound k's expression and
+    // the IBool of constraint j (in cons_id order) on a box, evaluated exactly as
+    // apply and the row checks evaluate them
+    body << "    template<class Num> __device__ static IVT<Num> bound_value(int k, const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+    body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";
+    for(std::size_t k = 0; k < p.bounds.size(); ++k) {
+        body << "        case " << k << ": {\n" << e.eval(p.bounds[k].expr, "            ");
+        body << "            return " << e.v(p.bounds[k].expr) << ";\n        }\n";
+    }
+    body << "        default: return IV{Num(1), Num(0), true};\n        }\n    }\n";
+    body << "    template<class Num> __device__ static IB cons_value(int j, const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+    body << "        switch(j) {\n";
+    for(std::size_t j = 0; j < cons.size(); ++j) body << "        case " << j << ": return c" << cons[j] << "(lb, ub, C);\n";
+    body << "        default: return IB{false, true};\n        }\n    }\n";
     // bound application
     body << "    template<class Num> __device__ static int apply(int k, Num (&lb)[N], Num (&ub)[N], const Num* C) {\n";
     body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";

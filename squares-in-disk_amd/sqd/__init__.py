@@ -90,15 +90,17 @@ class Progress(ctypes.Structure):
 
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Progress))
+CRITICAL_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.c_size_t, ctypes.c_size_t)
 
 
 class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("mode", ctypes.c_int32), ("level", ctypes.c_int32),
-                ("launches", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
+                ("launches", ctypes.c_uint64), ("milliseconds", ctypes.c_double), ("children", ctypes.c_uint64)]
 
     def as_dict(self):
         return {"name": self.name.decode(), "mode": self.mode, "level": self.level, "launches": int(self.launches),
-                "milliseconds": self.milliseconds}
+                "milliseconds": self.milliseconds, "children": int(self.children)}
 
 
 _P = ctypes.c_void_p
@@ -159,6 +161,9 @@ def lib():
             "sqd_test_acos_fallbacks": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_core_acos_fallbacks": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_core_set_progress": (ctypes.c_int, [_P, PROGRESS_FN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+            "sqd_core_set_critical_sink": (ctypes.c_int, [_P, CRITICAL_FN, ctypes.c_void_p]),
+            "sqd_eval_program": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _SZ, _D, ctypes.POINTER(ctypes.c_uint8),
+                                                ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
             "sqd_core_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_width_stats": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(WidthStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_last_error": (ctypes.c_char_p, []),
@@ -184,7 +189,7 @@ def exported_symbols():
             "sqd_service_wait", "sqd_service_destroy",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
-            "sqd_core_set_progress",
+            "sqd_core_set_progress", "sqd_core_set_critical_sink", "sqd_eval_program",
             "sqd_rounded_op", "sqd_interval_op",
             "sqd_test_acos_fallbacks", "sqd_last_error"]
 
@@ -262,6 +267,7 @@ class Core:
     def __init__(self, program, device=0, settings=None):
         self.program = program
         self.N = program.nargs
+        self._crit = None          # the critical sink's callback, while one is registered
         self._h = _P()
         s = settings if settings is not None else default_settings()
         _check(lib().sqd_core_create(device, program._h, ctypes.byref(s), ctypes.byref(self._h)))
@@ -314,7 +320,7 @@ class Core:
         c = Counts()
         _check(lib().sqd_final(self._h, _dptr(lb), _dptr(ub), n, _dptr(cl), _dptr(cu), capacity, ctypes.byref(k),
                                 ctypes.byref(c)))
-        m = min(k.value, capacity)
+        m = 0 if self._crit is not None else min(k.value, capacity)   # a sink received them
         return cl[:, :m].copy(), cu[:, :m].copy(), c.as_dict()
 
     def split(self, lb, ub, capacity=1 << 20):
@@ -335,7 +341,7 @@ class Core:
         k = _SZ()
         secs = ctypes.c_double()
         _check(lib().sqd_prove(self._h, ctypes.byref(c), None, None, 0, ctypes.byref(k), ctypes.byref(secs)))
-        m = min(k.value, critical_capacity)
+        m = 0 if self._crit is not None else min(k.value, critical_capacity)   # a sink received them
         cl = np.zeros((self.N, m))
         cu = np.zeros((self.N, m))
         if m:
@@ -399,6 +405,23 @@ class Core:
         self._progress = PROGRESS_FN(cb)              # kept alive while registered
         _check(lib().sqd_core_set_progress(self._h, self._progress, None, start_after, every))
 
+    def set_critical_sink(self, fn):
+        """fn(lb, ub), arrays of shape (N, n), for every chunk of criticals that
+        prove, prove_range and final find (None: off); those calls then return no
+        critical boxes themselves."""
+        if fn is None:
+            self._crit = None
+            _check(lib().sqd_core_set_critical_sink(self._h, CRITICAL_FN(), None))
+            return
+        n_args = self.N
+
+        def cb(_user, lb, ub, n, stride):
+            a = np.ctypeslib.as_array(lb, shape=(n_args * stride,)).reshape(n_args, stride)[:, :n].copy()
+            b = np.ctypeslib.as_array(ub, shape=(n_args * stride,)).reshape(n_args, stride)[:, :n].copy()
+            fn(a, b)
+        self._crit = CRITICAL_FN(cb)                  # kept alive while registered
+        _check(lib().sqd_core_set_critical_sink(self._h, self._crit, None))
+
     def shape(self):
         """(initial subdivisions of the first dynamic variable, static levels)."""
         a, b = ctypes.c_uint32(), ctypes.c_uint32()
@@ -436,6 +459,23 @@ class Core:
         _check(lib().sqd_width_stats(self._h, _dptr(lb), _dptr(ub), n, arr, k.value, ctypes.byref(k)))
         return [{"constraint": arr[i].constraint, "samples": int(arr[i].samples), "mean_ratio": arr[i].mean_ratio,
                  "max_ratio": arr[i].max_ratio} for i in range(k.value)]
+
+    def eval_program(self, lb, ub, precision=64):
+        """Bound and constraint values on boxes (SoA arrays of shape (N, n)):
+        (bounds (n, n_bounds, 3) = lb, ub, undefined; constraints (n, n_cons, 2) =
+        definitely, possibly), in binary64 or (precision=32) binary32."""
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        nb, nc = _SZ(), _SZ()
+        _check(lib().sqd_eval_program(self._h, precision, _dptr(lb), _dptr(ub), n, None, None, ctypes.byref(nb),
+                                      ctypes.byref(nc)))
+        b = np.zeros((n, nb.value, 3))
+        c = np.zeros((n, nc.value, 2), dtype=np.uint8)
+        _check(lib().sqd_eval_program(self._h, precision, _dptr(lb), _dptr(ub), n, _dptr(b),
+                                      c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(nb),
+                                      ctypes.byref(nc)))
+        return b, c
 
     def set_profiling(self, enable=True, only=None):
         """Time launches with HIP events on the core's stream (clears the statistics).
