@@ -13,6 +13,7 @@
 //      bound_dependency_analysis/cbd_*.hpp
 //  * split policy processing   prover_input/process_var_splitting.hpp:30-139
 #include "lower.hpp"
+#include "lower_passes.hpp"
 #include "bounds_rules.hpp"
 #include <unordered_map>
 #include <map>
@@ -433,13 +434,12 @@ flat::Program lower_constraint_system(const ConstraintSystem& cs, const std::vec
     }
     // RCT rows
     p.rows.assign(static_cast<std::size_t>(N), {});
-    for(const auto& c : cons) {
-        int id = fl.go(c);
-        int na = num_args(c);
-        bool unrewritten = true;
-        for(int a = 0; a < N && unrewritten; ++a) if(!rewrite(a, c).empty()) unrewritten = false;
-        if(na >= 1 && na < N && unrewritten) p.rows[static_cast<std::size_t>(na - 1)].push_back(id);
-        p.rows[static_cast<std::size_t>(N - 1)].push_back(id);
+    {
+        std::vector<int> ids;   // flattened in constraint order
+        for(const auto& c : cons) ids.push_back(fl.go(c));
+        const auto rows = passes::constraint_rows(cons, N);
+        for(std::size_t a = 0; a < rows.size(); ++a)
+            for(int i : rows[a]) p.rows[a].push_back(ids[static_cast<std::size_t>(i)]);
     }
     // compile-time box and its outward double enclosure
     for(int a = 0; a < N; ++a) {
@@ -454,5 +454,38 @@ flat::Program lower_constraint_system(const ConstraintSystem& cs, const std::vec
     }
     return p;
 }
+
+namespace passes {
+
+NodeP bound_and_simplify(const NodeP& n, const std::vector<fixed_point_bounds::Bounds>& arg_bounds) {
+    BoundAndSimplify bs{arg_bounds, {}};
+    return bs.go(n);
+}
+
+std::vector<BoundRewrite> rewrite_to_bounds(int target, const NodeP& constraint) {
+    std::vector<BoundRewrite> out;
+    for(const Rewrite& r : rewrite(target, constraint)) out.push_back(BoundRewrite{r.target, r.dir, r.expr});
+    return out;
+}
+
+BoundDependencies bound_dependencies(const NodeP& bounded_expr, int arg) {
+    const Deps d = deps(bounded_expr, arg);
+    return BoundDependencies{d.ll, d.lu, d.ul, d.uu};
+}
+
+std::vector<std::vector<int>> constraint_rows(const std::vector<NodeP>& constraints, int nargs) {
+    std::vector<std::vector<int>> rows(static_cast<std::size_t>(nargs));
+    for(std::size_t i = 0; i < constraints.size(); ++i) {
+        const NodeP& c = constraints[i];
+        const int na = num_args(c);
+        bool unrewritten = true;
+        for(int a = 0; a < nargs && unrewritten; ++a) if(!rewrite(a, c).empty()) unrewritten = false;
+        if(na >= 1 && na < nargs && unrewritten) rows[static_cast<std::size_t>(na - 1)].push_back(static_cast<int>(i));
+        rows[static_cast<std::size_t>(nargs - 1)].push_back(static_cast<int>(i));
+    }
+    return rows;
+}
+
+} // namespace passes
 
 } // namespace ivarp
