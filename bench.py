@@ -33,6 +33,29 @@ checked against the published ones.  The line printed carries
               cuboids/s, p50/p99 job latency and ratio to `value`, the fraction
               of static-level children the FP32 pass decided, and the
               FP32-versus-FP64 width ratio of each constraint on sampled boxes
+  ratio_decomposition
+              the batched leg's ratio split into its causes, on One and on
+              Four, every configuration through the same service shape
+              (`--workers` x `--batch-jobs`): plain binary64 in batches (no
+              group test, no prefilter), the binary64 group test in batches,
+              and the FP32-first service (autotuned); ratios to the plain
+              batched rate
+  legs        the other configurations of the metric, each with its counts
+              checked against the published ones, cuboids/s, p50/p99, a
+              roofline of its dominant kernel (binary64 operations per child
+              from the op-counting restatement on the CPU sample below, times
+              the children per launch the device counted, over the mean launch
+              time) and a CPU baseline on a stated sample (a contiguous range
+              of the first variable's initial subdivisions whose counts equal
+              the device's on the same range):
+                four_fp32_first   Four Subcontainer Lemma (37,245,328 cuboids),
+                                  plain binary64 proofs and the FP32-first
+                                  service with autotuned settings, the FP32
+                                  decided fraction and width ratios
+                more1_fp32_first  >=5 Subcontainers, s_n <= sigma, y_3 <= 0
+                                  (1,056,154,896 cuboids) with HIP-graph
+                                  rounds and the frontier in HBM, plain
+                                  binary64 and FP32-first (autotuned)
   job_stream  proof jobs over all 11 statements (statement uniform, a
               contiguous 1/2^k slice of the first variable's initial
               subdivisions, k uniform in 0..6, seeded) through a service with a
@@ -62,8 +85,12 @@ SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4   # MI355X: 256 CUs x 4 SIMDs; a w
 PUBLISHED = {                    # (cuboids, leaves) from the reference's prover output
     "top1": (128, 128), "top2": (168, 168), "one": (2869888, 2858678), "two": (7181696, 7069587),
     "three": (3549856, 3440786), "four": (37245328, 35026303), "five": (3392, 3188), "six": (14848, 13928),
-    "seven": (1808, 1703),
+    "seven": (1808, 1703), "more1": (1056154896, 1022752013), "more2": (123127968, 115680431),
 }
+# published wall time of the reference's multithreaded CPU prover (unstated CPU, about 8 threads),
+# prover_output_23_11_2020.txt:8-34
+PUBLISHED_SECONDS = {"one": 1.13986, "four": 34.0994, "more1": 4068.32}
+FP32_PEAK_TFLOPS = 157.3         # MI355X binary32 vector peak
 WORKLOADS = {
     "one": "one_subcontainer_plain_fp64",
     "two": "two_subcontainer_plain_fp64",
@@ -95,6 +122,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the Four and More1 legs")
+    ap.add_argument("--no-decomposition", action="store_true", help="skip the batched-ratio decomposition")
+    ap.add_argument("--leg-proofs", type=int, default=64, help="batched Four proofs per configuration")
+    ap.add_argument("--cpu-sample-cuboids", type=float, default=6e6,
+                    help="about this many cuboids in the CPU sample of the Four and More1 legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB: a kernel of known traffic that "
                          "scales the profiler's FETCH_SIZE/WRITE_SIZE readings")
@@ -268,6 +300,221 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
     return out
 
 
+def service_batches(sqd, prog, settings, expected, workers, batch_jobs, proofs):
+    """`proofs` whole proofs of `prog` through a service whose cores use `settings`
+    (`workers` workers, `batch_jobs` proofs per batch); every proof's counts are
+    checked.  Cuboids/s over the submitted proofs and p50/p99 job latency."""
+    n = initial_subdivisions(prog)
+    with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
+        svc.set_settings(prog, settings)
+        for _ in range(2):                                    # creates and warms every worker's core
+            warm = [svc.submit(prog, 0, n) for _ in range(workers * batch_jobs)]
+            for i in warm:
+                svc.wait(i)
+        t0 = time.perf_counter()
+        ids = [svc.submit(prog, 0, n) for _ in range(proofs)]
+        res = [svc.wait(i) for i in ids]
+        elapsed = time.perf_counter() - t0
+    for r in res:
+        c = r["counts"]
+        if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+            raise SystemExit("count mismatch in a batched run: %s" % c)
+    lat = [r["latency_seconds"] for r in res]
+    return {"value": sum(r["counts"]["num_cuboids"] for r in res) / elapsed, "unit": "cuboids/s",
+            "proofs": proofs, "seconds": elapsed,
+            "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3}}
+
+
+def service_settings(sqd, batch_jobs, **kw):
+    """Settings of a service core (the service's buffer sizes) with the given fields."""
+    import ctypes
+    o = sqd.ServiceOptions()
+    sqd.lib().sqd_service_options_default(ctypes.byref(o))
+    s = sqd.default_settings(jobs=batch_jobs, level_bytes=o.level_bytes, frontier_capacity=o.frontier_capacity)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def decompose(sqd, key, workers, batch_jobs, proofs, fp32_first=None, fp32_settings=None):
+    """Plain binary64, binary64 group test and FP32-first, all in batches of the
+    same service shape, on whole proofs of `key`; fp32_first: an already measured
+    FP32-first result (with its settings) to reuse, else tuned and run here."""
+    prog = sqd.Program.from_lemma(key)
+    expected = PUBLISHED[key]
+    plain = service_batches(sqd, prog, service_settings(sqd, batch_jobs, fp32_prefilter=0, group_levels=0), expected,
+                            workers, batch_jobs, proofs)
+    grouped = service_batches(sqd, prog, service_settings(sqd, batch_jobs, fp32_prefilter=0, group_levels=0xFFFFFFFF,
+                                                          prefilter_group=16), expected, workers, batch_jobs, proofs)
+    if fp32_first is None:
+        if fp32_settings is None:
+            fp32_settings, _ = sqd.autotune(prog, base=service_settings(sqd, batch_jobs), fp32_first=True)
+        fp32_first = service_batches(sqd, prog, fp32_settings, expected, workers, batch_jobs, proofs)
+        fp32_first["settings"] = {k: getattr(fp32_settings, k) for k in
+                                  ("fp32_prefilter", "group_levels", "prefilter_group", "dequeue_buffer_size",
+                                   "grid_blocks")}
+    return {
+        "lemma": key, "workers": workers, "batch_jobs": batch_jobs, "proofs": proofs,
+        "plain_batched": plain, "grouped_binary64_batched": dict(grouped, settings={"group_levels": "all",
+                                                                                     "prefilter_group": 16}),
+        "fp32_first_batched": fp32_first,
+        "ratio_grouped_vs_plain_batched": grouped["value"] / plain["value"],
+        "ratio_vs_plain_batched": fp32_first["value"] / plain["value"],
+        "ratio_fp32_first_vs_grouped": fp32_first["value"] / grouped["value"],
+    }
+
+
+def profiled_proof(sqd, core, expected):
+    """One profiled proof: the kernel statistics (time, launches, children) of every kind."""
+    core.set_profiling(True)
+    r = core.prove()
+    stats = core.kernel_stats()
+    core.set_profiling(False)
+    c = r["counts"]
+    if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+        raise SystemExit("count mismatch in a profiled proof: %s" % c)
+    return stats
+
+
+def dominant(stats):
+    return max((s for s in stats if s["mode"] >= 0), key=lambda s: s["milliseconds"])
+
+
+def timed_proofs(core, expected, steps):
+    lat, total = [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        r = core.prove()
+        lat.append(time.perf_counter() - s0)
+        c = r["counts"]
+        if (c["num_cuboids"], c["num_leaf_cuboids"]) != expected or r["n_critical"]:
+            raise SystemExit("count mismatch: %s" % c)
+        total += c["num_cuboids"]
+    elapsed = time.perf_counter() - t0
+    return {"value": total / elapsed, "unit": "cuboids/s", "proofs": steps, "seconds": elapsed,
+            "latency_ms": {"p50": percentile(lat, 0.5) * 1e3, "p99": percentile(lat, 0.99) * 1e3}}
+
+
+def sample_range(sqd, prog, target):
+    """The contiguous range of the first variable's initial subdivisions whose
+    proof has the cuboid count closest to `target` (in ratio; per-subdivision
+    counts from the device; ties go to the first range)."""
+    import math
+    n = initial_subdivisions(prog)
+    with sqd.Core(prog) as core:
+        per = [core.prove_range(i, i + 1)["counts"]["num_cuboids"] for i in range(n)]
+    pre = [0]
+    for v in per:
+        pre.append(pre[-1] + v)
+    best = None
+    for a in range(n):
+        for b in range(a + 1, n + 1):
+            t = pre[b] - pre[a]
+            if t <= 0:
+                continue
+            d = abs(math.log(t / target))
+            if best is None or d < best[0]:
+                best = (d, a, b, t)
+    return best[1], best[2], best[3]
+
+
+def oracle_sample(sqd, prog, rng, fp32=False):
+    """The CPU restatement on the initial range `rng` (None: the whole proof): a
+    timed run and an op-counting run (binary32 too with fp32)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.NamedTemporaryFile("w", suffix=".prog", delete=False) as f:
+        f.write(prog.text())
+        path = f.name
+    try:
+        timed = pyoracle.run_cli(path, threads=threads, init_range=rng, timeout=900)
+        counted = pyoracle.run_cli(path, threads=threads, init_range=rng, flops=True, timeout=900)
+        counted32 = pyoracle.run_cli(path, threads=threads, init_range=rng, flops=True, fp32=True,
+                                     timeout=900) if fp32 else None
+    finally:
+        os.unlink(path)
+    return threads, timed, counted, counted32
+
+
+def leg_roofline(stat, phases, fp32=False):
+    """Roofline of a level kernel kind: operations per child from the op-counted
+    restatement (phase "mode.level": [binary64 ops, children, binary32 ops]) times
+    the children per launch the device counted, over the mean launch time."""
+    ph = phases.get("%d.%d" % (stat["mode"], stat["level"]))
+    if not ph or not ph[1] or not stat["launches"] or not stat["children"]:
+        return None
+    mean_ms = stat["milliseconds"] / stat["launches"]
+    children = stat["children"] / stat["launches"]
+    f64 = ph[0] / ph[1] * children
+    f32 = ph[2] / ph[1] * children if fp32 else 0.0
+    t = mean_ms * 1e-3
+    a64, a32 = f64 / t / 1e12, f32 / t / 1e12
+    frac = a64 / FP64_PEAK_TFLOPS + a32 / FP32_PEAK_TFLOPS       # share of the launch the ops need at peak
+    peak = (a64 + a32) / frac if frac else FP64_PEAK_TFLOPS
+    return {"bound": "valu", "achieved": a64 + a32, "peak": peak, "unit": "TFLOP/s", "frac": frac,
+            "traffic": None, "kernel": stat["name"], "mean_launch_ms": mean_ms,
+            "children_per_launch": children, "flops_per_launch": f64 + f32,
+            "binary64_ops_per_child": ph[0] / ph[1], "binary32_ops_per_child": ph[2] / ph[1] if fp32 else 0.0,
+            "achieved_binary64": a64, "achieved_binary32": a32}
+
+
+def lemma_leg(sqd, key, plain_steps, sample_cuboids, workers, batch_jobs, proofs, cpu=True, batched=True):
+    """One configuration of the metric beyond One (see the module docstring)."""
+    prog = sqd.Program.from_lemma(key)
+    expected = PUBLISHED[key]
+    out = {"lemma": key, "cuboids_per_proof": expected[0], "leaves_per_proof": expected[1],
+           "reference_published_seconds": PUBLISHED_SECONDS.get(key)}
+    # plain binary64, reference split policy, HIP-graph rounds, frontier in HBM
+    with sqd.Core(prog, settings=sqd.default_settings(generation_count=8)) as core:
+        core.prove()
+        stats = profiled_proof(sqd, core, expected)
+        core.prove()                                          # captures the round graphs
+        out["plain_fp64"] = timed_proofs(core, expected, plain_steps)
+    dom = dominant(stats)
+    out["plain_fp64"]["kernel_ms_per_proof"] = {s["name"]: s["milliseconds"] for s in
+                                                sorted(stats, key=lambda s: -s["milliseconds"])[:4]}
+    # FP32-first on a core of one job (autotuned), HIP-graph rounds
+    best, tune = sqd.autotune(prog, base=sqd.default_settings(), fp32_first=True)
+    with sqd.Core(prog, settings=best) as core:
+        core.prove()
+        stats32 = profiled_proof(sqd, core, expected)
+        checked, decided = core.fp32_stats()
+        core.prove()
+        out["fp32_first"] = timed_proofs(core, expected, plain_steps)
+    dom32 = dominant(stats32)
+    out["fp32_first"].update({
+        "settings": {k: tune["settings"][k] for k in ("fp32_prefilter", "group_levels", "prefilter_group",
+                                                      "dequeue_buffer_size", "grid_blocks")},
+        "fp32_checked": checked, "fp32_decided": decided,
+        "fp32_decided_fraction": decided / checked if checked else None,
+        "kernel_ms_per_proof": {s["name"]: s["milliseconds"] for s in
+                                sorted(stats32, key=lambda s: -s["milliseconds"])[:4]},
+        "ratio_vs_plain_fp64": None})
+    out["fp32_first"]["ratio_vs_plain_fp64"] = out["fp32_first"]["value"] / out["plain_fp64"]["value"]
+    out["value"] = out["fp32_first"]["value"]
+    if batched:
+        out["batched"] = decompose(sqd, key, workers, batch_jobs, proofs)
+        out["value"] = max(out["value"], out["batched"]["fp32_first_batched"]["value"])
+    out["width_ratio_per_constraint"] = width_sample(sqd, prog, n_boxes=4096)
+    if cpu:
+        a, b, gpu_cuboids = sample_range(sqd, prog, sample_cuboids)
+        with sqd.Core(prog) as core:
+            g = core.prove_range(a, b)["counts"]
+        threads, timed, counted, counted32 = oracle_sample(sqd, prog, (a, b), fp32=True)
+        if (timed["num_cuboids"], timed["num_leaf_cuboids"]) != (g["num_cuboids"], g["num_leaf_cuboids"]):
+            raise SystemExit("%s: oracle sample differs from the device on [%d, %d)" % (key, a, b))
+        out["cpu_baseline"] = {
+            "value": timed["num_cuboids"] / timed["seconds"], "unit": "cuboids/s", "cores": threads, "kind": "port",
+            "sample": "initial subdivisions [%d, %d) of the first variable: %d cuboids (equal to the device's), "
+                      "%.2f s with %d threads" % (a, b, timed["num_cuboids"], timed["seconds"], threads)}
+        out["roofline"] = leg_roofline(dom, counted["phases"])
+        out["roofline_fp32_first"] = leg_roofline(dom32, counted32["phases"], fp32=bool(best.fp32_prefilter))
+        out["op_count_sample"] = "the same range, op-counting restatement"
+    return out
+
+
 def width_sample(sqd, prog, n_boxes=8192, seed=3):
     """FP32-versus-FP64 width ratio per constraint on boxes of the static grid."""
     import numpy as np
@@ -391,6 +638,21 @@ def main():
                                                            a.batched_proofs, value, counted32, a.batched_settings)
     if batched is not None:
         batched["width_ratio_per_constraint"] = width_sample(sqd, prog)
+    decomposition = {}
+    if batched is not None and not a.no_decomposition:
+        mine = {k: batched[k] for k in ("value", "unit", "proofs", "seconds", "latency_ms")}
+        mine["settings"] = batched.get("autotuned", batched.get("settings"))
+        decomposition[a.lemma] = decompose(sqd, a.lemma, a.workers, a.batch_jobs, a.batched_proofs, fp32_first=mine)
+        batched["ratio_vs_plain_batched"] = decomposition[a.lemma]["ratio_vs_plain_batched"]
+    legs = {}
+    if not a.no_legs:
+        legs["four_fp32_first"] = lemma_leg(sqd, "four", 10, a.cpu_sample_cuboids, a.workers, a.batch_jobs,
+                                            a.leg_proofs, cpu=not a.no_cpu_baseline,
+                                            batched=not a.no_decomposition)
+        if "batched" in legs["four_fp32_first"]:
+            decomposition["four"] = legs["four_fp32_first"]["batched"]
+        legs["more1_fp32_first"] = lemma_leg(sqd, "more1", 3, a.cpu_sample_cuboids, a.workers, a.batch_jobs,
+                                             0, cpu=not a.no_cpu_baseline, batched=False)
     stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.stream_batch_jobs, a.seed)
 
     dom = next(s for s in stats if s["mode"] == dom["mode"] and s["level"] == dom["level"])
@@ -436,6 +698,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_first_batched": batched,
+        "ratio_decomposition": decomposition or None,
+        "legs": legs or None,
         "job_stream": stream,
     }
     print(json.dumps(line), flush=True)

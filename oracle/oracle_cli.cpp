@@ -1,5 +1,9 @@
 // Command-line driver of the CPU oracle (test infrastructure only).
-//   sqd_oracle <program-file> [--threads T] [--generations G] [--flops] [--slice FIRST LAST] [--fp32]
+//   sqd_oracle <program-file> [--threads T] [--generations G] [--flops] [--slice FIRST LAST]
+//              [--range FIRST LAST] [--fp32]
+// --slice proves the initial-queue entries [FIRST, LAST); --range proves from the
+// first dynamic variable's initial subdivisions [FIRST, LAST) only (the unit of
+// sqd_prove_range).
 // Prints one JSON line with the counts, the wall time, the thread count,
 // with --fp32 the FP32 prefilter's checked/decided children and, with --flops,
 // the binary64 (and binary32) operation counts (total and per phase).
@@ -20,13 +24,15 @@ int main(int argc, char** argv) {
     bool flops = false;
     bool slice = false;
     bool fp32 = false;
-    unsigned long long first = 0, last = 0;
+    bool range = false;
+    unsigned long long first = 0, last = 0, r_first = 0, r_last = 0;
     for(int i = 2; i < argc; ++i) {
         if(!std::strcmp(argv[i], "--threads") && i + 1 < argc) threads = std::atoi(argv[++i]);
         else if(!std::strcmp(argv[i], "--generations") && i + 1 < argc) gens = std::atoi(argv[++i]);
         else if(!std::strcmp(argv[i], "--flops")) flops = true;
         else if(!std::strcmp(argv[i], "--fp32")) fp32 = true;
         else if(!std::strcmp(argv[i], "--slice") && i + 2 < argc) { slice = true; first = std::strtoull(argv[++i], nullptr, 10); last = std::strtoull(argv[++i], nullptr, 10); }
+        else if(!std::strcmp(argv[i], "--range") && i + 2 < argc) { range = true; r_first = std::strtoull(argv[++i], nullptr, 10); r_last = std::strtoull(argv[++i], nullptr, 10); }
     }
     std::ifstream f(argv[1]);
     std::stringstream ss;
@@ -34,6 +40,7 @@ int main(int argc, char** argv) {
     void* h = nullptr;
     if(oracle_load(ss.str().c_str(), &h) != 0) { std::fprintf(stderr, "load: %s\n", oracle_last_error()); return 1; }
     oracle_set_fp32_prefilter(h, fp32 ? 1 : 0);
+    if(range && oracle_set_init_range(h, r_first, r_last) != 0) { std::fprintf(stderr, "bad --range\n"); return 2; }
     oracle_flop_enable(flops ? 1 : 0);
     oracle_flop_reset();
     OracleCounts c{};

@@ -206,8 +206,12 @@ def interval_op(op, a, b=None):
     return out
 
 
-def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=None, timeout=None, fp32=False):
-    """Run the oracle CLI on a program file; returns the parsed JSON line."""
+def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=None, timeout=None, fp32=False,
+            init_range=None):
+    """Run the oracle CLI on a program file; returns the parsed JSON line.
+
+    slice_range: initial-queue entries [first, last); init_range: the first
+    dynamic variable's initial subdivisions [first, last) (as sqd_prove_range)."""
     import json
     ensure_built()
     cmd = [str(CLI), str(program_path), "--threads", str(threads), "--generations", str(generations)]
@@ -217,5 +221,7 @@ def run_cli(program_path, threads=8, generations=8, flops=False, slice_range=Non
         cmd.append("--fp32")
     if slice_range is not None:
         cmd += ["--slice", str(slice_range[0]), str(slice_range[1])]
+    if init_range is not None:
+        cmd += ["--range", str(init_range[0]), str(init_range[1])]
     r = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=timeout)
     return json.loads(r.stdout.strip().splitlines()[-1])
