@@ -409,6 +409,25 @@ int sqd_interval_op(int device, int op, size_t n, const double* a, const double*
  * decided at 2^-113 precision (results widened instead); 0 in every run so far. */
 int sqd_test_acos_fallbacks(int device, uint64_t* n);
 
+/* ---------------------------------------------------------------- without allocation */
+
+/* sqd_prove_range without a core and without allocating device memory: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  One job
+ * of `program` proves the initial range [first, last) with `settings` (NULL:
+ * the defaults; jobs and graphs are ignored) exactly as sqd_prove_range does,
+ * its device memory laid out in a static 64 MiB array that each device holds
+ * until the process ends (SQD_ERR_CAPACITY, before any work, when the settings
+ * need more: lower level_bytes, frontier_capacity, critical_capacity).  The
+ * call allocates no device or pinned memory, no stream and no event; it runs
+ * on the device's null stream.  Calls on one device share the array and are
+ * serialised by a lock (overlapping calls would race on it); each call clears
+ * what must start at zero.  Outputs as sqd_prove_range (criticals up to
+ * capacity, SoA stride capacity; *n_critical the total). */
+int sqd_prove_range_noalloc(int device, const sqd_program* program, const sqd_settings* settings, uint32_t first,
+                            uint32_t last, sqd_counts* counts, double* crit_lb, double* crit_ub, size_t capacity,
+                            size_t* n_critical);
+
 const char* sqd_last_error(void);
 
 #ifdef __cplusplus

@@ -1062,6 +1062,31 @@ namespace sqd {
 void set_last_error(const std::string& message) { t_error = message; }
 }
 
+// ---------------------------------------------------------------- scratch
+// Device memory of sqd_prove_range_noalloc: one job's layout (control block,
+// frontier, level buffers, lists, hull records, constants) placed in this
+// array instead of an allocation.  Every device the library runs on holds its
+// own copy until the process ends.  Host code never passes the array's name to
+// a kernel or a copy: its device address is read once per device
+// (hipGetSymbolAddress) and kept in g_scratch.
+constexpr std::size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace sqd {
+namespace {
+constexpr int kScratchDevices = 64;
+// Per device: the lock that serialises the calls sharing its scratch array,
+// the array's device address (set once) and the pageable host copy of the
+// control block.
+struct ScratchDevice {
+    std::mutex m;
+    char* ptr = nullptr;
+    DeviceState host_state;
+};
+ScratchDevice g_scratch[kScratchDevices];
+} // namespace
+} // namespace sqd
+
 using namespace sqd;
 
 extern "C" {
@@ -1530,6 +1555,53 @@ int sqd_core_kernel_stats(const sqd_core* c, sqd_kernel_stat* out, size_t capaci
             if(!out || i >= capacity) break;
             out[i++] = kv.second;
         }
+    });
+}
+
+int sqd_prove_range_noalloc(int device, const sqd_program* program, const sqd_settings* settings, uint32_t first,
+                            uint32_t last, sqd_counts* counts, double* crit_lb, double* crit_ub, size_t capacity,
+                            size_t* n_critical) {
+    return guarded([&] {
+        if(!program) throw std::invalid_argument("null program");
+        int n_dev = 0;
+        hip_check(hipGetDeviceCount(&n_dev), "hipGetDeviceCount");
+        if(device < 0 || device >= n_dev || device >= kScratchDevices) throw std::invalid_argument("no such device");
+        ScratchDevice& sd = g_scratch[device];
+        std::lock_guard<std::mutex> lock(sd.m);
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        if(!sd.ptr) {   // set-up, once per device, with the device current
+            void* p = nullptr;
+            hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+            if(!p) throw Failure(SQD_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave no address");
+            sd.ptr = static_cast<char*>(p);
+        }
+        sqd_settings s;
+        if(settings) s = *settings;
+        else sqd_settings_default(&s);
+        s.jobs = 1;     // one job's layout
+        s.graphs = 0;   // a graph would be an allocation
+        sqd_core c;     // host-side state only; its stream is the device's null stream
+        Layout lay;
+        unsigned long long* state_words = nullptr;
+        configure_core(&c, device, program, &s, lay, state_words);
+        if(c.job_stride > kScratchBytes)
+            throw Failure(SQD_ERR_CAPACITY, "these settings need " + std::to_string(c.job_stride) +
+                                                " bytes of device memory, the scratch array of sqd_prove_range_noalloc "
+                                                "holds " + std::to_string(kScratchBytes) +
+                                                "; lower level_bytes, frontier_capacity or critical_capacity");
+        lay.place(sd.ptr);
+        bind_state(&c, state_words);
+        c.h_state = &sd.host_state;
+        upload_job_constants(&c);
+        // begin_call clears the hull records and the control block (kernels, as
+        // the array keeps the last call's contents)
+        prove_jobs(&c, 1, &first, &last, nullptr, false);
+        const DeviceState& h = c.h_state[0];
+        const unsigned long long k = h.crit_count;
+        if(counts) *counts = counts_of(h);
+        if(n_critical) *n_critical = k;
+        if(crit_lb && crit_ub && k)
+            download_soa(&c, c.crit, std::min<unsigned long long>({k, capacity, c.crit.cap}), crit_lb, crit_ub, capacity);
     });
 }
 

@@ -164,6 +164,9 @@ def lib():
             "sqd_core_set_critical_sink": (ctypes.c_int, [_P, CRITICAL_FN, ctypes.c_void_p]),
             "sqd_eval_program": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _SZ, _D, ctypes.POINTER(ctypes.c_uint8),
                                                 ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+            "sqd_prove_range_noalloc": (ctypes.c_int, [ctypes.c_int, _P, ctypes.POINTER(Settings), ctypes.c_uint32,
+                                                       ctypes.c_uint32, ctypes.POINTER(Counts), _D, _D, _SZ,
+                                                       ctypes.POINTER(_SZ)]),
             "sqd_core_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_width_stats": (ctypes.c_int, [_P, _D, _D, _SZ, ctypes.POINTER(WidthStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_last_error": (ctypes.c_char_p, []),
@@ -189,7 +192,7 @@ def exported_symbols():
             "sqd_service_wait", "sqd_service_destroy",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
-            "sqd_core_set_progress", "sqd_core_set_critical_sink", "sqd_eval_program",
+            "sqd_core_set_progress", "sqd_core_set_critical_sink", "sqd_prove_range_noalloc", "sqd_eval_program",
             "sqd_rounded_op", "sqd_interval_op",
             "sqd_test_acos_fallbacks", "sqd_last_error"]
 
