@@ -17,6 +17,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <set>
 #include <vector>
 
 #include <mpfr.h>
@@ -875,6 +876,42 @@ Cuboid soa_get(const Program& p, std::uint64_t n, const double* lb, const double
 
 } // namespace
 
+namespace {
+template<class X>
+void eval_program_boxes(const Program& p, std::uint64_t n, const double* lb, const double* ub, double* bv,
+                        std::uint8_t* cv) {
+    std::set<int> roots;
+    for(const auto& r : p.rows) for(int c : r) roots.insert(c);
+    const std::vector<int> cons(roots.begin(), roots.end());
+    EvaluatorT<X> ev(p);
+    CuboidT<X> c;
+    c.lb.resize(static_cast<std::size_t>(p.N));
+    c.ub.resize(static_cast<std::size_t>(p.N));
+    const std::size_t nb = p.bounds.size(), nc = cons.size();
+    for(std::uint64_t i = 0; i < n; ++i) {
+        for(int a = 0; a < p.N; ++a) {
+            const double l = lb[static_cast<std::uint64_t>(a) * n + i], u = ub[static_cast<std::uint64_t>(a) * n + i];
+            if(sizeof(X) == 4) { c.lb[static_cast<std::size_t>(a)] = static_cast<X>(f32_down(l)); c.ub[static_cast<std::size_t>(a)] = static_cast<X>(f32_up(u)); }
+            else { c.lb[static_cast<std::size_t>(a)] = static_cast<X>(l); c.ub[static_cast<std::size_t>(a)] = static_cast<X>(u); }
+        }
+        for(std::size_t k = 0; k < nb; ++k) {
+            const IVT<X> v = ev.eval_bound(static_cast<int>(k), c);
+            double* o = bv + (i * nb + k) * 3;
+            o[0] = static_cast<double>(v.lb);
+            o[1] = static_cast<double>(v.ub);
+            o[2] = v.und ? 1.0 : 0.0;
+        }
+        for(std::size_t j = 0; j < nc; ++j) {
+            ev.begin();
+            ev.need(cons[j], c);
+            const IB b = ev.pb[static_cast<std::size_t>(cons[j])];
+            cv[(i * nc + j) * 2] = b.def ? 1 : 0;
+            cv[(i * nc + j) * 2 + 1] = b.poss ? 1 : 0;
+        }
+    }
+}
+} // namespace
+
 extern "C" {
 
 int oracle_load(const char* text, void** handle) {
@@ -1123,6 +1160,17 @@ int oracle_phase_counts32(int mode, int level, std::uint64_t* flops) {
     const int ph = mode == 4 ? kPhaseOther : mode * kPhaseLevels + level;
     *flops = g_phase_flops32[ph].load();
     return 0;
+}
+
+int oracle_eval_program(void* handle, int precision, std::uint64_t n, const double* lb, const double* ub,
+                        double* bound_values, std::uint8_t* cons_values) {
+    return guarded([&] {
+        if(precision != 64 && precision != 32) throw std::invalid_argument("precision must be 64 or 32");
+        RoundDownGuard g;
+        const Program& p = static_cast<Handle*>(handle)->prog;
+        if(precision == 32) eval_program_boxes<float>(p, n, lb, ub, bound_values, cons_values);
+        else eval_program_boxes<double>(p, n, lb, ub, bound_values, cons_values);
+    });
 }
 
 int oracle_set_init_range(void* handle, std::uint64_t first, std::uint64_t last) {

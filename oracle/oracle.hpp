@@ -105,4 +105,14 @@ int oracle_set_fp32_prefilter(void* handle, int on);
 int oracle_set_init_range(void* handle, std::uint64_t first, std::uint64_t last);
 // Static-level children of the last computing call that binary32 evaluated / pruned.
 int oracle_fp32_stats(void* handle, std::uint64_t* checked, std::uint64_t* decided);
+// Bound and constraint values on n boxes (SoA, stride n), as the device's
+// sqd_eval_program gives them: bound k's expression on box i to
+// bound_values[(i * n_bounds + k) * 3 + {lb, ub, undefined}], the IBool of
+// constraint j (the constraint-row roots in ascending node order) to
+// cons_values[(i * n_cons + j) * 2 + {definitely, possibly}]; precision 32
+// evaluates on the box converted outward to binary32, as the prefilter does.
+// The evaluation is the one the bound applications and row checks use
+// (EvaluatorT::eval_bound / possibly).
+int oracle_eval_program(void* handle, int precision, std::uint64_t n, const double* lb, const double* ub,
+                        double* bound_values, std::uint8_t* cons_values);
 }

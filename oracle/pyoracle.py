@@ -67,6 +67,7 @@ def lib():
             "oracle_set_fp32_prefilter": (ctypes.c_int, [_P, ctypes.c_int]),
             "oracle_set_init_range": (ctypes.c_int, [_P, _U64, _U64]),
             "oracle_fp32_stats": (ctypes.c_int, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+            "oracle_eval_program": (ctypes.c_int, [_P, ctypes.c_int, _U64, _D, _D, _D, ctypes.POINTER(ctypes.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -144,6 +145,18 @@ class Oracle:
         _check(lib().oracle_initial_queue(self._h, _dp(lb), _dp(ub), cap, ctypes.byref(n), ctypes.byref(c)))
         k = min(n.value, cap)
         return lb[: k * self.N].reshape(k, self.N).T.copy(), ub[: k * self.N].reshape(k, self.N).T.copy(), c.as_dict()
+
+    def eval_program(self, lb, ub, n_bounds, n_constraints, precision=64):
+        """Bound values (n, n_bounds, 3) and constraint IBools (n, n_constraints, 2)
+        on SoA boxes (N, n), as sqd_eval_program gives them."""
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        n = lb.shape[1]
+        b = np.zeros((n, n_bounds, 3))
+        c = np.zeros((n, n_constraints, 2), dtype=np.uint8)
+        _check(lib().oracle_eval_program(self._h, precision, n, _dp(lb), _dp(ub), _dp(b),
+                                         c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return b, c
 
     def nonfinal(self, lb, ub):
         lb = np.ascontiguousarray(lb, dtype=np.float64)

@@ -259,10 +259,10 @@ class Program:
                         r = _sqrt(a)
                     elif op == "acos":
                         r = _acos(a)
-                    elif op == "max":
-                        r = V(max(a.lo, b.lo), max(a.hi, b.hi))
+                    elif op == "max":                          # n-ary
+                        r = V(max(x.lo for x in args), max(x.hi for x in args))
                     elif op == "min":
-                        r = V(min(a.lo, b.lo), min(a.hi, b.hi))
+                        r = V(min(x.lo for x in args), min(x.hi for x in args))
                     else:
                         raise ValueError("unknown node kind " + op)
             memo[i] = r
