@@ -174,9 +174,28 @@ typedef struct sqd_job_counts {
     uint64_t n_critical;      /* criticals the job found (sqd_core_job_criticals) */
     uint64_t fp32_checked;    /* static-level children the FP32 prefilter evaluated */
     uint64_t fp32_decided;    /* ... and pruned */
+    uint64_t frontier_peak;   /* most cuboids the job's frontier held after a round */
 } sqd_job_counts;
 int sqd_prove_batch(sqd_core* core, size_t n_jobs, const uint32_t* first, const uint32_t* last,
                     sqd_job_counts* results, double* seconds);
+/* Continuous batches on a core of sqd_settings.jobs rows (no counterpart in the
+ * reference, which proves one statement at a time): jobs join free rows at
+ * round boundaries and leave when their frontier is empty, so that a job
+ * arriving while others run starts with the next round instead of waiting for
+ * the whole batch.  sqd_batch_admit puts the job [first, last) (as in
+ * sqd_prove_range) into the free row `row`; it starts with the next step.
+ * sqd_batch_step runs one round on every row that holds a job (after the initial
+ * queues of the rows admitted since the last step, only theirs): *finished
+ * receives the mask of rows whose job ended in this step (their counts in
+ * results[row], which needs sqd_settings.jobs entries; their criticals stay
+ * available to sqd_core_job_criticals until the row starts another job), *active
+ * the number of rows still running.  Each job's counts and criticals are exactly
+ * those of its range proved alone.  A failing step (a buffer too small for one
+ * job) drops every row's job; any other computing call on the core ends the
+ * continuous batch the same way. */
+int sqd_batch_admit(sqd_core* core, uint32_t row, uint32_t first, uint32_t last);
+int sqd_batch_step(sqd_core* core, sqd_job_counts* results, uint64_t* finished, uint32_t* active);
+
 /* The critical cuboids of job `job` of the last sqd_prove_batch (as
  * sqd_core_criticals: up to capacity boxes, SoA stride capacity; *n the total). */
 int sqd_core_job_criticals(sqd_core* core, size_t job, double* crit_lb, double* crit_ub, size_t capacity,
@@ -315,9 +334,11 @@ int sqd_autotune(int device, const sqd_program* program, const sqd_settings* bas
 /* A queue of proof jobs (a program and a range [first, last) of its first
  * variable's initial subdivisions, as in sqd_prove_range) run by worker threads,
  * each on a pooled core of the job's program with its own stream.  A worker
- * takes the oldest queued job together with up to batch_jobs - 1 later queued
- * jobs of the same program and proves them as one batch (sqd_prove_batch); the
- * batches of up to `workers` workers overlap on the device.  Cores are created on
+ * takes the program of the oldest queued job and runs a continuous batch of it
+ * (sqd_batch_admit / sqd_batch_step): queued jobs of the program fill the core's
+ * batch_jobs rows, a job leaves its row when its frontier is empty, and jobs
+ * that arrive meanwhile join at the next round boundary; the batches of up to
+ * `workers` workers overlap on the device.  Cores are created on
  * demand within `memory_budget` bytes of device memory; idle cores of other
  * programs are destroyed to make room.  Jobs may be submitted from any thread
  * while others run.  Programs must outlive the service.  No counterpart in the
@@ -344,7 +365,8 @@ typedef struct sqd_job_result {
     double run_seconds;           /* start to completion */
     double latency_seconds;       /* submission to completion */
     int32_t status;               /* SQD_OK or the job's error */
-    uint32_t reserved;
+    uint32_t joined_running;      /* 1: the job joined a batch that was already running */
+    uint64_t frontier_peak;       /* most cuboids its frontier held after a round */
 } sqd_job_result;
 void sqd_service_options_default(sqd_service_options* options);
 int sqd_service_create(const sqd_service_options* options, sqd_service** out);

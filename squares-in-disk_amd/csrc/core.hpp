@@ -577,7 +577,12 @@ void k_static_level(LevelArgs job0) {
     const unsigned long long total =
         F32 ? (listed < args.undec_cap ? listed : args.undec_cap) : n_items * static_cast<unsigned long long>(SUBS);
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(F32 && static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // block-uniform
+    // a block without inputs leaves at once (block-uniform: no barrier is skipped
+    // by part of a block, and it has no counts to add); the hoisting path hands
+    // out whole parents per wave
+    if(!F32 && wave_hoisting<P, L>() ? static_cast<unsigned long long>(blockIdx.x) * (blockDim.x / 64) >= n_items
+                                     : static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total)
+        return;
     // the generated code takes the constants as literals (P::kc) and does not
     // read the array pointer its functions take
     const double* const s_c = nullptr;
@@ -689,6 +694,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
     const int step = SUBS / cpl;                                                       // groups per parent
     const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // no groups in any trip (block-uniform)
     const unsigned long long trips = (total + stride - 1) / stride;
     __shared__ unsigned long long s_groups[kBlock];   // the trip's groups that the group test left
     __shared__ unsigned long long s_ngroups;
@@ -826,6 +832,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_group64(LevelA
     const int step = SUBS / cpl;                                                       // groups per parent
     const unsigned long long total = n_items * static_cast<unsigned long long>(step);  // groups
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // no groups (block-uniform)
     const unsigned long long trips = (total + stride - 1) / stride;
     unsigned long long c_cub = 0, c_leaf = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {
@@ -889,6 +896,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_dynamic_level(LevelAr
     const unsigned long long total = n_items * r_n;
     const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
     const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(static_cast<unsigned long long>(blockIdx.x) * blockDim.x >= total) return;   // no children (block-uniform)
     const unsigned long long trips = (total + stride - 1) / stride;
     unsigned long long c_cub = 0, c_leaf = 0;
     for(unsigned long long trip = 0; trip < trips; ++trip) {

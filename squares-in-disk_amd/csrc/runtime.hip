@@ -80,19 +80,58 @@ static_assert(sizeof(DeviceState) % sizeof(unsigned long long) == 0, "state is a
 __global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
     if(threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
+// Start of a proof: the job's control block to zero and its initial runtime
+// box (computed once when the core was created, as the reference computes it
+// when it builds the prover input, refactor_constraints_impl.hpp:114) as the
+// single entry of the initial-queue input list, at depth 0.  Row blockIdx.y
+// works on job blockIdx.y.  With `start_flag` (a word per job) only the jobs
+// whose flag is set start; the others keep their state and get an empty
+// initial-queue input.
+__global__ void k_proof_start(unsigned long long* words, unsigned n_words, const double* box, int n_args,
+                              double* list_lb, double* list_ub, unsigned* list_depth, unsigned long long list_stride,
+                              unsigned long long count_word, const unsigned long long* start_flag,
+                              unsigned long long stride) {
+    const unsigned long long off = dev::job_offset(stride);
+    words = dev::job_ptr(words, off);
+    box = dev::job_ptr(box, off);
+    list_lb = dev::job_ptr(list_lb, off);
+    list_ub = dev::job_ptr(list_ub, off);
+    list_depth = dev::job_ptr(list_depth, off);
+    start_flag = dev::job_ptr(start_flag, off);
+    if(blockIdx.x != 0) return;
+    if(start_flag && *start_flag == 0ull) {
+        if(threadIdx.x == 0) words[count_word] = 0ull;
+        return;
+    }
+    for(unsigned i = threadIdx.x; i < n_words; i += blockDim.x) words[i] = 0ull;
+    for(int a = static_cast<int>(threadIdx.x); a < n_args; a += static_cast<int>(blockDim.x)) {
+        list_lb[static_cast<unsigned long long>(a) * list_stride] = box[a];
+        list_ub[static_cast<unsigned long long>(a) * list_stride] = box[n_args + a];
+    }
+    if(threadIdx.x == 0 && list_depth) list_depth[0] = 0u;
+    __syncthreads();
+    if(threadIdx.x == 0) words[count_word] = 1ull;
+}
 // Most jobs of one batch (sqd_prove_batch).
 constexpr unsigned kMaxJobs = 64;
-// The initial-queue range of each job of a batch: [lo[j], hi[j]).
+// The initial-queue range of each job of a batch: [lo[j], hi[j]), and whether
+// the job starts with the next proof start (continuous batches start only the
+// rows that were admitted since the last round).
 struct JobRanges {
     unsigned lo[kMaxJobs];
     unsigned hi[kMaxJobs];
+    unsigned start[kMaxJobs];
 };
 // Kernels below with a `stride` argument run one grid row per job of a batch:
 // row blockIdx.y works on job blockIdx.y's memory (job 0's pointers shifted by
 // blockIdx.y * stride bytes).
 __global__ void k_set_ranges(unsigned long long* p, unsigned long long stride, JobRanges r) {
     p = dev::job_ptr(p, dev::job_offset(stride));
-    if(threadIdx.x == 0 && blockIdx.x == 0) { p[0] = r.lo[blockIdx.y]; p[1] = r.hi[blockIdx.y]; }
+    if(threadIdx.x == 0 && blockIdx.x == 0) {
+        p[0] = r.lo[blockIdx.y];
+        p[1] = r.hi[blockIdx.y];
+        p[2] = r.start[blockIdx.y];
+    }
 }
 // Start of a proof (or of a host-driven call): the whole control block (state
 // and per-block count slots) to zero.
@@ -309,6 +348,13 @@ struct sqd_core {
     bool fp32 = false;               // some static level runs the FP32-prefiltered kernels
     // sqd_prove as HIP graphs: the initial queue with the first round, and one later round
     hipGraphExec_t g_first = nullptr, g_round = nullptr;
+    // continuous batches (sqd_batch_admit / sqd_batch_step): per row 0 free, 1 admitted
+    // (starts with the next step), 2 running; the rows' ranges; the most cuboids each
+    // row's frontier held after a round; graphs of a step with and without starting rows
+    std::vector<unsigned char> row_state;
+    std::vector<unsigned> row_first, row_last;
+    std::vector<unsigned long long> row_peak;
+    hipGraphExec_t g_cont_start = nullptr, g_cont_round = nullptr;
     unsigned proofs = 0;             // sqd_prove calls so far (graphs are captured from the second)
     unsigned long long* d_range = nullptr;   // initial-queue range of the first variable's subdivisions
     unsigned long long* d_undec = nullptr;   // FP32 prefilter: undecided child indices of one level
@@ -317,7 +363,7 @@ struct sqd_core {
     unsigned long long split_fanout = 1;
     std::vector<std::uint32_t> st_subs, dyn_subs, dyn_initial;
     std::vector<double> box_init;                  // host copy of the compile-time box (uploaded per job)
-    unsigned long long range_init[2] = {0, 1};     // host copy of the full initial range
+    unsigned long long range_init[3] = {0, 1, 1};  // host copy of the full initial range (and start flag)
     // profiling: events around each launch, resolved at the next stream sync
     struct Pending { int kind; hipEvent_t a, b; };
     bool profiling = false;
@@ -335,15 +381,21 @@ namespace {
 enum : int { KIND_HULL_FINALIZE = 100, KIND_FINAL_NOSTATIC, KIND_INIT_HULL, KIND_HULL_DECODE, KIND_INITIAL_BOX,
              KIND_SET_COUNT };
 
-sqd_kernel_stat make_stat(int kind, bool fp32) {
+// A static level's kernels are named after how the core decides it: the
+// binary32 prefilter with binary64 for the undecided children, the binary64
+// group test, or the plain binary64 pass.
+sqd_kernel_stat make_stat(int kind, const sqd_settings& settings) {
     sqd_kernel_stat s{};
     static const char* modes[] = {"NONFINAL", "FINAL", "SPLIT", "INIT"};
     if(kind < 64) {
         s.mode = kind / 16;
         s.level = kind % 16;
-        std::snprintf(s.name, sizeof s.name, "%s<%s,%d>",
-                      s.mode <= 1 ? (fp32 ? "static_level_fp32" : "static_level") : "dynamic_level",
-                      modes[s.mode], s.level);
+        const char* kernel = "dynamic_level";
+        if(s.mode <= 1)
+            kernel = ((settings.fp32_prefilter >> s.level) & 1u) ? "static_level_fp32"
+                   : ((settings.group_levels >> s.level) & 1u)  ? "static_level_group64"
+                                                                 : "static_level";
+        std::snprintf(s.name, sizeof s.name, "%s<%s,%d>", kernel, modes[s.mode], s.level);
     } else {
         s.mode = -1;
         s.level = -1;
@@ -388,7 +440,7 @@ void sync(sqd_core* c) {
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
         auto it = c->stats.find(p.kind);
-        if(it == c->stats.end()) it = c->stats.emplace(p.kind, make_stat(p.kind, c->fp32)).first;
+        if(it == c->stats.end()) it = c->stats.emplace(p.kind, make_stat(p.kind, c->settings)).first;
         it->second.launches += 1;
         it->second.milliseconds += ms;
     }
@@ -624,6 +676,8 @@ void download_soa(sqd_core* c, const DevList& l, unsigned long long n, double* l
 // with begin_call and marks the core clean when it completes; after a failed
 // call both are reset in full before the next use.
 void begin_call(sqd_core* c) {
+    // any other computing call ends a continuous batch: its rows' jobs are dropped
+    std::fill(c->row_state.begin(), c->row_state.end(), static_cast<unsigned char>(0));
     if(c->clean < c->ny) {
         if(c->ks->NSTAT > 0)
             launch(c, KIND_INIT_HULL, "k_init_hull", [&] {
@@ -675,9 +729,20 @@ void set_range(sqd_core* c, unsigned long long first, unsigned long long last) {
 unsigned long long full_range(const sqd_core* c) { return c->dyn_initial.empty() ? 1ull : c->dyn_initial[0]; }
 
 // Enqueues a cleared device state and the initial queue on the (empty) stack;
-// nothing is waited for.
-void enqueue_initial_queue(sqd_core* c) {
-    launch_initial_box(c, true, true);
+// nothing is waited for.  The initial box is the one computed when the core
+// was created (d_box_init).
+void enqueue_initial_queue(sqd_core* c, bool only_flagged = false) {
+    launch(c, KIND_INITIAL_BOX, "proof start", [&] {
+        const unsigned words = static_cast<unsigned>(sizeof(DeviceState) / sizeof(unsigned long long));
+        const unsigned long long count_word =
+            static_cast<unsigned long long>(reinterpret_cast<char*>(c->scratch.count) - reinterpret_cast<char*>(c->d_state)) /
+            sizeof(unsigned long long);
+        hipLaunchKernelGGL(k_proof_start, dim3(1, c->ny), dim3(64), 0, c->stream,
+                           reinterpret_cast<unsigned long long*>(c->d_state), words, c->d_box_init, c->N, c->scratch.lb,
+                           c->scratch.ub, c->scratch.depth, c->scratch.cap, count_word,
+                           only_flagged ? c->d_range + 2 : nullptr, c->job_stride);
+        return hipGetLastError();
+    });
     run_dynamic(c, dev::INIT, c->scratch.view(), c->scratch.count, 1, c->stack.view());
 }
 
@@ -838,6 +903,12 @@ void drain_criticals(sqd_core* c, unsigned long long k, unsigned long long n_fin
     set_count(c, c->crit.count, 0);
 }
 
+// The most cuboids each job's frontier held after a round (rows of the last state copy).
+void track_peaks(sqd_core* c) {
+    if(c->row_peak.size() < c->ny) c->row_peak.resize(c->ny, 0ull);
+    for(unsigned j = 0; j < c->ny; ++j) c->row_peak[j] = std::max(c->row_peak[j], c->h_state[j].stack_count);
+}
+
 // After a round of a proof on a core of one job: hands the round's criticals to
 // the sink, if one is set.
 void drain_round(sqd_core* c) {
@@ -874,6 +945,7 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
     set_ranges(c, r);
     begin_call(c);
     c->crit_delivered = 0;
+    c->row_peak.assign(c->ny, 0ull);
     const bool graphs = c->settings.graphs && !c->profiling && c->proofs > 0;
     if(graphs && !c->g_first) {
         c->g_first = capture_graph(c, [&] { enqueue_initial_queue(c); enqueue_round(c); enqueue_state_copy(c); });
@@ -894,12 +966,14 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
     if(graphs) {
         hip_check(hipGraphLaunch(c->g_first, c->stream), "hipGraphLaunch");
         finish_state(c);
+        track_peaks(c);
         drain_round(c);
         ++rounds;
         while(frontier_left(c)) {
             progress();
             hip_check(hipGraphLaunch(c->g_round, c->stream), "hipGraphLaunch");
             finish_state(c);
+            track_peaks(c);
             drain_round(c);
             ++rounds;
         }
@@ -909,6 +983,7 @@ void prove_jobs(sqd_core* c, unsigned n, const uint32_t* first, const uint32_t* 
             if(rounds > 0) progress();
             enqueue_round(c);
             read_state(c);
+            track_peaks(c);
             count_children(c);
             drain_round(c);
             ++rounds;
@@ -1001,7 +1076,7 @@ void configure_core(sqd_core* c, int device, const sqd_program* program, const s
     c->crit.lay(lay, c->N, s.critical_capacity, false, false);
     c->scratch.lay(lay, c->N, std::max<unsigned long long>(kmax, 1024), true, false);
     lay.take(c->d_hull, static_cast<std::size_t>(kmax) * c->N * 4);
-    lay.take(c->d_range, 2);
+    lay.take(c->d_range, 3);   // first, last, start flag
     if(c->fp32 || (s.group_levels & used)) {
         c->undec_cap = undec_cap;
         lay.take(c->d_undec, static_cast<std::size_t>(undec_cap));
@@ -1019,6 +1094,20 @@ void configure_core(sqd_core* c, int device, const sqd_program* program, const s
     if(kmax > c->fin.cap || kmax > c->spl.cap || kmax * worst_static > lvl_cap || kmax * worst_split > lvl_cap ||
        (c->undec_cap && kmax * worst_undec > c->undec_cap))
         throw Failure(SQD_ERR_INTERNAL, "core sizing rule violated");
+}
+
+// The initial runtime box of every job (refactor_constraints_impl.hpp:61-101),
+// once per core: it depends on the program alone.
+void compute_initial_boxes(sqd_core* c) {
+    const unsigned rows = c->ny;
+    c->ny = c->jobs;
+    try {
+        launch_initial_box(c, false, false);
+    } catch(...) {
+        c->ny = rows;
+        throw;
+    }
+    c->ny = rows;
 }
 
 // Device pointers into the control block once the job memory is placed.
@@ -1040,6 +1129,7 @@ void upload_job_constants(sqd_core* c) {
     const auto& prog = c->program->prog;
     c->range_init[0] = 0ull;
     c->range_init[1] = c->dyn_initial.empty() ? 1ull : c->dyn_initial[0];
+    c->range_init[2] = 1ull;
     c->box_init.assign(prog.box_lb.begin(), prog.box_lb.end());
     c->box_init.insert(c->box_init.end(), prog.box_ub.begin(), prog.box_ub.end());
     const std::size_t n_consts = 2 * static_cast<std::size_t>(std::max(1, c->ks->NC));
@@ -1179,6 +1269,7 @@ int sqd_core_create(int device, const sqd_program* program, const sqd_settings* 
         c->h_state = static_cast<DeviceState*>(hs);
         std::memset(c->h_state, 0, sizeof(DeviceState) * c->jobs);
         upload_job_constants(c.get());
+        compute_initial_boxes(c.get());
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->device_bytes = t_dalloc_bytes - bytes0;
         *out = c.release();
@@ -1195,6 +1286,8 @@ void sqd_core_destroy(sqd_core* c) {
     if(c->h_state) (void)hipHostFree(c->h_state);
     if(c->g_first) (void)hipGraphExecDestroy(c->g_first);
     if(c->g_round) (void)hipGraphExecDestroy(c->g_round);
+    if(c->g_cont_start) (void)hipGraphExecDestroy(c->g_cont_start);
+    if(c->g_cont_round) (void)hipGraphExecDestroy(c->g_cont_round);
     for(hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1204,7 +1297,6 @@ int sqd_initial_box(sqd_core* c, double* lb, double* ub) {
     return guarded([&] {
         if(!c || !lb || !ub) throw std::invalid_argument("null argument");
         hip_check(hipSetDevice(c->device), "hipSetDevice");
-        launch_initial_box(c, false, false);
         sync(c);
         hip_check(hipMemcpyAsync(lb, c->d_box_init, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream), "copy");
         hip_check(hipMemcpyAsync(ub, c->d_box_init + c->N, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream), "copy");
@@ -1361,7 +1453,97 @@ int sqd_prove_batch(sqd_core* c, size_t n_jobs, const uint32_t* first, const uin
             results[j].n_critical = c->h_state[j].crit_count;
             results[j].fp32_checked = c->h_state[j].f32_checked;
             results[j].fp32_decided = c->h_state[j].f32_decided;
+            results[j].frontier_peak = j < c->row_peak.size() ? c->row_peak[j] : 0ull;
         }
+    });
+}
+
+int sqd_batch_admit(sqd_core* c, uint32_t row, uint32_t first, uint32_t last) {
+    return guarded([&] {
+        if(!c) throw std::invalid_argument("null core");
+        if(row >= c->jobs) throw std::invalid_argument("row beyond the core's batch size");
+        if(first >= last || last > full_range(c))
+            throw std::invalid_argument("initial range must satisfy first < last <= the first variable's subdivisions");
+        if(c->row_state.size() != c->jobs) {
+            c->row_state.assign(c->jobs, 0);
+            c->row_first.assign(c->jobs, 0u);
+            c->row_last.assign(c->jobs, 0u);
+        }
+        if(c->row_state[row] != 0) throw std::invalid_argument("the row holds a job");
+        c->row_state[row] = 1;
+        c->row_first[row] = first;
+        c->row_last[row] = last;
+    });
+}
+
+int sqd_batch_step(sqd_core* c, sqd_job_counts* results, uint64_t* finished, uint32_t* active) {
+    return guarded([&] {
+        if(!c || !results || !finished || !active) throw std::invalid_argument("null argument");
+        *finished = 0;
+        *active = 0;
+        if(c->row_state.size() != c->jobs) return;   // nothing admitted yet
+        bool running = false, starting = false;
+        for(unsigned char st : c->row_state) { running = running || st == 2; starting = starting || st == 1; }
+        if(!running && !starting) return;
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        struct Rows {
+            sqd_core* c;
+            explicit Rows(sqd_core* core) : c(core) { c->ny = c->jobs; }
+            ~Rows() { c->ny = 1; }
+        } rows(c);
+        std::vector<unsigned char> state = c->row_state;   // begin_call below drops the rows' jobs
+        try {
+            if(!running) {
+                begin_call(c);                            // a fresh batch: reset what a failed call left
+                c->row_state = state;
+                c->row_peak.assign(c->jobs, 0ull);
+            } else if(c->clean < c->ny) {
+                throw Failure(SQD_ERR_INTERNAL, "continuous batch interrupted");
+            }
+            c->clean = 0;
+            if(starting) {
+                JobRanges r{};
+                for(unsigned j = 0; j < c->jobs; ++j) {
+                    r.lo[j] = c->row_first[j];
+                    r.hi[j] = c->row_last[j];
+                    r.start[j] = c->row_state[j] == 1 ? 1u : 0u;
+                    if(c->row_state[j] == 1) c->row_peak[j] = 0;
+                }
+                set_ranges(c, r);
+            }
+            if(c->settings.graphs && !c->profiling) {
+                hipGraphExec_t& g = starting ? c->g_cont_start : c->g_cont_round;
+                if(!g) g = capture_graph(c, [&] {
+                    if(starting) enqueue_initial_queue(c, true);
+                    enqueue_round(c);
+                    enqueue_state_copy(c);
+                });
+                hip_check(hipGraphLaunch(g, c->stream), "hipGraphLaunch");
+                finish_state(c);
+            } else {
+                if(starting) enqueue_initial_queue(c, true);
+                enqueue_round(c);
+                read_state(c);
+            }
+        } catch(...) {
+            std::fill(c->row_state.begin(), c->row_state.end(), static_cast<unsigned char>(0));
+            throw;
+        }
+        track_peaks(c);
+        for(unsigned j = 0; j < c->jobs; ++j) {
+            if(c->row_state[j] == 0) continue;
+            c->row_state[j] = 2;
+            const DeviceState& h = c->h_state[j];
+            if(h.stack_count > 0) { ++*active; continue; }
+            results[j].counts = counts_of(h);
+            results[j].n_critical = h.crit_count;
+            results[j].fp32_checked = h.f32_checked;
+            results[j].fp32_decided = h.f32_decided;
+            results[j].frontier_peak = c->row_peak[j];
+            *finished |= 1ull << j;
+            c->row_state[j] = 0;
+        }
+        c->clean = c->ny;   // a completed round leaves hull records and count slots reset
     });
 }
 
@@ -1593,6 +1775,7 @@ int sqd_prove_range_noalloc(int device, const sqd_program* program, const sqd_se
         bind_state(&c, state_words);
         c.h_state = &sd.host_state;
         upload_job_constants(&c);
+        compute_initial_boxes(&c);
         // begin_call clears the hull records and the control block (kernels, as
         // the array keeps the last call's contents)
         prove_jobs(&c, 1, &first, &last, nullptr, false);

@@ -7,7 +7,9 @@
 // streams map onto the device's hardware queues) while each job's frontier stays
 // in HBM.  Cores are pooled per program and created on demand within a device
 // memory budget; an idle core of another program is destroyed to make room.  A
-// job is admitted when a worker and a core (or the memory for one) are free.
+// worker runs a continuous batch of one program on a core: queued jobs of the
+// program take its free rows, and a job that arrives while the batch runs joins
+// at the next round boundary (sqd_batch_step).
 //
 // Autotuner: times proofs of a slice of the program's first variable (batches
 // of sqd_settings.jobs copies of it on a batch core) and picks, by coordinate
@@ -258,6 +260,7 @@ struct sqd_service {
         uint32_t first = 0, last = 0;
         Clock::time_point submitted, started, finished;
         bool done = false;
+        bool joined = false;   // admitted into a batch that was already running
         sqd_job_result result{};
         std::string error;
     };
@@ -331,25 +334,48 @@ struct sqd_service {
         }
     }
 
+    // With the lock held: the oldest queued job of `prog`, taken off the queue (0: none).
+    uint64_t take_queued(const sqd_program* prog) {
+        for(auto it = queue.begin(); it != queue.end(); ++it)
+            if(jobs[*it].prog == prog) {
+                const uint64_t id = *it;
+                queue.erase(it);
+                return id;
+            }
+        return 0;
+    }
+
+    // With the lock held: completes job `id` with its result.
+    void complete(uint64_t id, const sqd_job_counts& jc, int status, const std::string& error) {
+        Job& j = jobs[id];
+        j.finished = Clock::now();
+        sqd_job_result r{};
+        r.counts = jc.counts;
+        r.n_critical = jc.n_critical;
+        r.frontier_peak = jc.frontier_peak;
+        r.status = status;
+        r.joined_running = j.joined ? 1u : 0u;
+        r.queued_seconds = seconds_since(j.submitted, j.started);
+        r.run_seconds = seconds_since(j.started, j.finished);
+        r.latency_seconds = seconds_since(j.submitted, j.finished);
+        j.result = r;
+        j.error = error;
+        j.done = true;
+    }
+
+    // A worker takes the program of the oldest queued job, a core of that
+    // program, and runs a continuous batch on it: queued jobs of the program
+    // fill the core's free rows, the batch advances a round at a time
+    // (sqd_batch_step), a job whose frontier empties completes and frees its
+    // row, and jobs that arrived meanwhile join at the next round boundary.  The
+    // batch ends when no row holds a job.
     void run_worker(int device) {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(m);
         for(;;) {
             cv.wait(lk, [&] { return stop || !queue.empty(); });
             if(stop) return;
-            const uint64_t id = queue.front();
-            queue.pop_front();
-            const sqd_program* prog = jobs[id].prog;
-            // the batch: this job and the next queued ones of the same program
-            std::vector<uint64_t> batch{id};
-            for(auto it = queue.begin(); it != queue.end() && batch.size() < opt.batch_jobs;) {
-                if(jobs[*it].prog == prog) {
-                    batch.push_back(*it);
-                    it = queue.erase(it);
-                } else {
-                    ++it;
-                }
-            }
+            const sqd_program* prog = jobs[queue.front()].prog;
             // settings of the program (tuned once, by the first worker that needs them)
             if(!settings.count(prog)) {
                 if(opt.autotune && !tuning[prog]) {
@@ -371,69 +397,97 @@ struct sqd_service {
                     settings[prog] = base_settings();
                 }
             }
+            bool queued = false;   // another worker may have taken the program's jobs meanwhile
+            for(uint64_t id : queue) queued = queued || jobs[id].prog == prog;
+            if(!queued) continue;
             const uint64_t slot = acquire(lk, prog);
             if(slot == 0) return;
-            const size_t n = batch.size();
-            std::vector<uint32_t> first(n), last(n);
-            const Clock::time_point started = Clock::now();
-            for(size_t i = 0; i < n; ++i) {
-                jobs[batch[i]].started = started;
-                first[i] = jobs[batch[i]].first;
-                last[i] = jobs[batch[i]].last;
-            }
             sqd_settings s = settings[prog];
             s.jobs = opt.batch_jobs;
             sqd_core* core = find_slot(slot)->core;
-            lk.unlock();
-            int rc = SQD_OK;
-            std::string err;
             if(!core) {
-                rc = sqd_core_create(device, prog, &s, &core);
-                if(rc != SQD_OK) err = sqd_last_error();
+                lk.unlock();
+                const int rc = sqd_core_create(device, prog, &s, &core);
+                const std::string err = rc != SQD_OK ? sqd_last_error() : std::string();
+                lk.lock();
+                if(rc != SQD_OK) {                  // creation failed: give the room back, fail one job
+                    Slot* sl = find_slot(slot);
+                    bytes_in_use -= sl->bytes;
+                    pool.erase(pool.begin() + (sl - pool.data()));
+                    const uint64_t id = take_queued(prog);
+                    if(id) {
+                        jobs[id].started = Clock::now();
+                        complete(id, sqd_job_counts{}, rc, err);
+                    }
+                    cv.notify_all();
+                    continue;
+                }
             }
-            std::vector<sqd_job_counts> jc(n);
-            std::vector<int> status(n, rc);
-            std::vector<std::string> errors(n, err);
-            if(rc == SQD_OK) {
-                if(sqd_prove_batch(core, n, first.data(), last.data(), jc.data(), nullptr) != SQD_OK) {
-                    // a failing job fails its batch: prove the jobs one by one to tell which
-                    for(size_t i = 0; i < n; ++i) {
-                        status[i] = sqd_prove_batch(core, 1, &first[i], &last[i], &jc[i], nullptr);
-                        if(status[i] != SQD_OK) errors[i] = sqd_last_error();
+            const uint32_t rows = opt.batch_jobs;
+            std::vector<uint64_t> row_job(rows, 0);
+            std::vector<uint32_t> row_first(rows, 0), row_last(rows, 0);   // read without the lock
+            uint32_t running = 0;
+            auto admit = [&] {   // with the lock held: queued jobs of prog into free rows
+                const bool joining = running > 0;
+                for(uint32_t r = 0; r < rows; ++r) {
+                    if(row_job[r]) continue;
+                    const uint64_t id = take_queued(prog);
+                    if(!id) break;
+                    Job& j = jobs[id];
+                    j.started = Clock::now();
+                    j.joined = joining;
+                    if(sqd_batch_admit(core, r, j.first, j.last) != SQD_OK) {
+                        complete(id, sqd_job_counts{}, SQD_ERR_ARGUMENT, sqd_last_error());
+                        continue;
+                    }
+                    row_job[r] = id;
+                    row_first[r] = j.first;
+                    row_last[r] = j.last;
+                    ++running;
+                }
+            };
+            admit();
+            std::vector<sqd_job_counts> jc(rows);
+            while(running > 0) {
+                lk.unlock();
+                uint64_t finished = 0;
+                uint32_t active = 0;
+                const int rc = sqd_batch_step(core, jc.data(), &finished, &active);
+                std::vector<int> status(rows, SQD_OK);
+                std::vector<std::string> errors(rows);
+                if(rc != SQD_OK) {
+                    // one job's buffers overflowed and the step dropped every row's job:
+                    // prove each alone to tell which one failed
+                    finished = 0;
+                    for(uint32_t r = 0; r < rows; ++r) {
+                        if(!row_job[r]) continue;
+                        const uint32_t first = row_first[r], last = row_last[r];
+                        jc[r] = sqd_job_counts{};
+                        status[r] = sqd_prove_batch(core, 1, &first, &last, &jc[r], nullptr);
+                        if(status[r] != SQD_OK) errors[r] = sqd_last_error();
+                        finished |= 1ull << r;
                     }
                 }
+                lk.lock();
+                for(uint32_t r = 0; r < rows; ++r) {
+                    if(!row_job[r] || !((finished >> r) & 1ull)) continue;
+                    complete(row_job[r], jc[r], status[r], errors[r]);
+                    row_job[r] = 0;
+                    --running;
+                }
+                cv.notify_all();
+                admit();   // jobs that arrived while the round ran join at this boundary
             }
             uint64_t bytes = 0;
-            if(core) sqd_core_memory(core, &bytes);
-            lk.lock();
+            sqd_core_memory(core, &bytes);
             Slot* sl = find_slot(slot);          // busy slots are never removed by others
-            if(core) {
-                if(bytes) {
-                    bytes_in_use = bytes_in_use - sl->bytes + bytes;
-                    sl->bytes = bytes;
-                    core_bytes[prog] = bytes;
-                }
-                sl->core = core;
-                sl->busy = false;
-            } else {                              // creation failed: give the room back
-                bytes_in_use -= sl->bytes;
-                pool.erase(pool.begin() + (sl - pool.data()));
+            if(bytes) {
+                bytes_in_use = bytes_in_use - sl->bytes + bytes;
+                sl->bytes = bytes;
+                core_bytes[prog] = bytes;
             }
-            const Clock::time_point finished = Clock::now();
-            for(size_t i = 0; i < n; ++i) {
-                Job& j = jobs[batch[i]];
-                j.finished = finished;
-                sqd_job_result r{};
-                r.counts = jc[i].counts;
-                r.n_critical = jc[i].n_critical;
-                r.status = status[i];
-                r.queued_seconds = seconds_since(j.submitted, j.started);
-                r.run_seconds = seconds_since(j.started, j.finished);
-                r.latency_seconds = seconds_since(j.submitted, j.finished);
-                j.result = r;
-                j.error = errors[i];
-                j.done = true;
-            }
+            sl->core = core;
+            sl->busy = false;
             cv.notify_all();
         }
     }

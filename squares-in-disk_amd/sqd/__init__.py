@@ -68,18 +68,19 @@ class ServiceOptions(ctypes.Structure):
 
 class JobCounts(ctypes.Structure):
     _fields_ = [("counts", Counts), ("n_critical", ctypes.c_uint64), ("fp32_checked", ctypes.c_uint64),
-                ("fp32_decided", ctypes.c_uint64)]
+                ("fp32_decided", ctypes.c_uint64), ("frontier_peak", ctypes.c_uint64)]
 
 
 class JobResult(ctypes.Structure):
     _fields_ = [("counts", Counts), ("n_critical", ctypes.c_uint64), ("queued_seconds", ctypes.c_double),
                 ("run_seconds", ctypes.c_double), ("latency_seconds", ctypes.c_double), ("status", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32)]
+                ("joined_running", ctypes.c_uint32), ("frontier_peak", ctypes.c_uint64)]
 
     def as_dict(self):
         return {"counts": self.counts.as_dict(), "n_critical": int(self.n_critical),
                 "queued_seconds": self.queued_seconds, "run_seconds": self.run_seconds,
-                "latency_seconds": self.latency_seconds, "status": int(self.status)}
+                "latency_seconds": self.latency_seconds, "status": int(self.status),
+                "joined_running": bool(self.joined_running), "frontier_peak": int(self.frontier_peak)}
 
 
 class Progress(ctypes.Structure):
@@ -141,6 +142,9 @@ def lib():
             "sqd_prove_batch": (ctypes.c_int, [_P, _SZ, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.POINTER(JobCounts), ctypes.POINTER(ctypes.c_double)]),
             "sqd_core_job_criticals": (ctypes.c_int, [_P, _SZ, _D, _D, _SZ, ctypes.POINTER(_SZ)]),
+            "sqd_batch_admit": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+            "sqd_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(JobCounts), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_uint32)]),
             "sqd_core_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
             "sqd_core_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_autotune_options_default": (None, [ctypes.POINTER(AutotuneOptions)]),
@@ -185,7 +189,7 @@ def exported_symbols():
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_prove_range",
-            "sqd_prove_batch", "sqd_core_job_criticals",
+            "sqd_prove_batch", "sqd_core_job_criticals", "sqd_batch_admit", "sqd_batch_step",
             "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
             "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
             "sqd_service_get_settings", "sqd_service_set_settings",
@@ -377,7 +381,7 @@ class Core:
         for j in range(n):
             r = res[j]
             d = {"counts": r.counts.as_dict(), "n_critical": int(r.n_critical), "seconds": secs.value,
-                 "fp32_stats": (int(r.fp32_checked), int(r.fp32_decided))}
+                 "fp32_stats": (int(r.fp32_checked), int(r.fp32_decided)), "frontier_peak": int(r.frontier_peak)}
             m = min(int(r.n_critical), critical_capacity)
             if critical_capacity:
                 cl = np.zeros((self.N, m))
@@ -388,6 +392,26 @@ class Core:
                 d["crit_lb"], d["crit_ub"] = cl, cu
             out.append(d)
         return out
+
+    def batch_admit(self, row, first, last):
+        """Continuous batch: the job [first, last) takes the free row `row` and
+        starts with the next batch_step."""
+        _check(lib().sqd_batch_admit(self._h, row, first, last))
+
+    def batch_step(self, rows):
+        """One round of the continuous batch on a core of `rows` rows: a dict
+        {row: result} of the jobs that ended in it, and the rows still running."""
+        res = (JobCounts * rows)()
+        fin = ctypes.c_uint64()
+        act = ctypes.c_uint32()
+        _check(lib().sqd_batch_step(self._h, res, ctypes.byref(fin), ctypes.byref(act)))
+        out = {}
+        for r in range(rows):
+            if (fin.value >> r) & 1:
+                out[r] = {"counts": res[r].counts.as_dict(), "n_critical": int(res[r].n_critical),
+                          "fp32_stats": (int(res[r].fp32_checked), int(res[r].fp32_decided)),
+                          "frontier_peak": int(res[r].frontier_peak)}
+        return out, int(act.value)
 
     def set_progress(self, fn, start_after=0.0, every=0.0):
         """fn(dict) after each due round and at the end of sqd_prove (None: off)."""
