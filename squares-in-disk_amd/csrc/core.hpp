@@ -188,7 +188,28 @@ __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int 
 {
     constexpr int NB = P::NB;
     static_assert(NB <= 48, "at most 48 bounds (4-bit recursion counters in three words)");
-    if(NB == 0) return false;
+    if(NB == 0 || it_limit <= 0) return false;
+    // The queue starts with the bounds that depend on `arg` and is worked off
+    // in order.  Every call site passes a compile-time `arg`, so the initial
+    // bounds are known here: they are applied first, in queue order, without
+    // the queue.  While none of them changes the box nothing else is queued, and
+    // when all leave it unchanged the propagation is over (the common case).
+    // From the first change on, the queue below takes over in exactly the state
+    // the plain algorithm reaches there: the initial bounds after it still
+    // queued, and the dependents of its change queued after them.
+    // A queue that receives all NB bounds at once reads as empty (begin == end,
+    // see the ring below), and then nothing is applied.
+    int m = 0;
+    P::deps(arg * 3 + 2, [&](int) { ++m; });
+    if(m >= NB) return false;
+    int done = 0, r_first = 0;
+    P::deps(arg * 3 + 2, [&](int k) {
+        if(r_first != 0) return;
+        r_first = P::apply(k, lb, ub, C);
+        ++done;
+    });
+    if(r_first < 0) return true;
+    if(r_first == 0) return false;
     const int lane = threadIdx.x;
     int qb = 0, qe = 0;
     unsigned long long present = 0, unstable = 0;
@@ -202,7 +223,24 @@ __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int 
         if(++qe >= NB) qe = 0;
         return true;
     };
-    P::deps(arg * 3 + 2, [&](int a) { enqueue(a); });
+    auto handle = [&](int r) {
+        P::deps(r - 1, [&](int a) {
+            if(rec_get(rec, a) >= rec_limit) {
+                if(!((present >> a) & 1ull) && !((unstable >> a) & 1ull)) {
+                    s.ulist[nun][lane] = static_cast<unsigned char>(a);
+                    ++nun;
+                    unstable |= 1ull << a;
+                }
+            } else if(enqueue(a)) {
+                rec_inc(rec, a);
+            }
+        });
+    };
+    int seen = 0;
+    P::deps(arg * 3 + 2, [&](int a) {
+        if(seen++ >= done) enqueue(a);   // the initial bounds not yet applied
+    });
+    handle(r_first);
     for(int it = 0; it < it_limit; ++it) {
         while(qb != qe) {
             const int k = s.ring[qb][lane];
@@ -210,19 +248,7 @@ __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int 
             if(++qb >= NB) qb = 0;
             const int r = P::apply(k, lb, ub, C);
             if(r < 0) return true;
-            if(r > 0) {
-                P::deps(r - 1, [&](int a) {
-                    if(rec_get(rec, a) >= rec_limit) {
-                        if(!((present >> a) & 1ull) && !((unstable >> a) & 1ull)) {
-                            s.ulist[nun][lane] = static_cast<unsigned char>(a);
-                            ++nun;
-                            unstable |= 1ull << a;
-                        }
-                    } else if(enqueue(a)) {
-                        rec_inc(rec, a);
-                    }
-                });
-            }
+            if(r > 0) handle(r);
         }
         if(nun > 0) {
             for(int j = 0; j < nun; ++j) enqueue(s.ulist[j][lane]);

@@ -363,8 +363,16 @@ std::string generate(const flat::Program& p, const std::string& key) {
         for(int w = 0; w < 3; ++w) {
             const auto& l = p.update[static_cast<std::size_t>(a)][static_cast<std::size_t>(w)];
             if(l.empty()) continue;
+            // a bound listed twice is queued once (the queue skips members), so
+            // only its first occurrence is emitted: propagate() applies the
+            // initial list in order before it builds the queue
+            std::vector<int> seen_once;
             body << "        case " << a * 3 + w << ":";
-            for(int x : l) body << " f(" << x << ");";
+            for(int x : l) {
+                if(std::find(seen_once.begin(), seen_once.end(), x) != seen_once.end()) continue;
+                seen_once.push_back(x);
+                body << " f(" << x << ");";
+            }
             body << " break;\n";
         }
     }
