@@ -117,7 +117,7 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=16, help="proofs per batch (one grid row each)")
     ap.add_argument("--stream-batch-jobs", type=int, default=4,
                     help="jobs per batch in the job-stream leg (11 programs share the memory budget)")
-    ap.add_argument("--batched-proofs", type=int, default=1024)
+    ap.add_argument("--batched-proofs", type=int, default=4096)
     ap.add_argument("--stream-jobs", type=int, default=48)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-batched", action="store_true")
@@ -271,7 +271,9 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
     with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
         svc.set_settings(prog, best)                          # the settings reported above
         for _ in range(2):                                    # creates and warms every worker's core
-            warm = [svc.submit(prog, 0, n) for _ in range(workers * batch_jobs)]
+            # deep enough that every worker starts a batch (a program gets a
+            # further batch while a whole batch of its jobs waits)
+            warm = [svc.submit(prog, 0, n) for _ in range(4 * workers * batch_jobs)]
             for i in warm:
                 svc.wait(i)
         t0 = time.perf_counter()
@@ -308,7 +310,9 @@ def service_batches(sqd, prog, settings, expected, workers, batch_jobs, proofs):
     with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
         svc.set_settings(prog, settings)
         for _ in range(2):                                    # creates and warms every worker's core
-            warm = [svc.submit(prog, 0, n) for _ in range(workers * batch_jobs)]
+            # deep enough that every worker starts a batch (a program gets a
+            # further batch while a whole batch of its jobs waits)
+            warm = [svc.submit(prog, 0, n) for _ in range(4 * workers * batch_jobs)]
             for i in warm:
                 svc.wait(i)
         t0 = time.perf_counter()

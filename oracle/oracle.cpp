@@ -247,11 +247,74 @@ IV interval_acos(IV x, CT B) {
     if(B.ub > kOne && x.ub > 1.0) { x.ub = 1.0; und = true; }
     return IV{acos_dir(x.ub, false), acos_dir(x.lb, true), und};
 }
-// binary32: the binary64 directed results converted outward (the largest float
-// below RD64(acos x) is the largest float below acos x)
+// binary32 (the FP32 prefilter's own definition, not the reference's): an
+// enclosure of acos in binary64 interval arithmetic, converted outward.  The
+// arcsine series P(z) = sum_k c_k z^k, c_k = binom(2k, k) / (4^k (2k + 1)), on
+// z = t^2 (|t| <= 1/2: acos t = pi/2 - t P(t^2)) or on z = u = (1 - |t|) / 2
+// (acos|t| = 2 sqrt(u) P(u), acos t = pi - acos|t| for t < 0), 16 terms with
+// every operation rounded outward, plus 2 c_16 4^-16 for the tail (z <= 1/4).
+// The same operations in the same order as the device (csrc/acos_enclosure.hpp).
+constexpr int kAcosTerms = 16;
+constexpr double acos_coeff(int k, bool up) {
+    unsigned __int128 num = 1, den = 1;
+    for(int i = 1; i <= k; ++i) num = num * static_cast<unsigned __int128>(k + i) / static_cast<unsigned __int128>(i);
+    for(int i = 0; i < k; ++i) den *= 4;
+    den *= static_cast<unsigned __int128>(2 * k + 1);
+    int sh = 0;
+    while((num << sh) < (den << 52)) ++sh;
+    const unsigned __int128 sc = num << sh;
+    unsigned long long q = static_cast<unsigned long long>(sc / den);
+    if(up && sc % den != 0) ++q;
+    double r = static_cast<double>(q);
+    for(int i = 0; i < sh; ++i) r *= 0.5;
+    return r;
+}
+inline double rd_fma(double a, double b, double c) { count(1); return opq(std::fma(opq(a), opq(b), opq(c))); }
+inline double ru_fma(double a, double b, double c) { return -rd_fma(-a, b, -c); }
+void acos_enclose(double t, double& lo, double& hi) {
+    static const double kHalfPiLo = 0x1.921fb54442d18p+0, kHalfPiHi = 0x1.921fb54442d19p+0;
+    static const double kPiLo = 0x1.921fb54442d18p+1, kPiHi = 0x1.921fb54442d19p+1;
+    static double clo[kAcosTerms], chi[kAcosTerms], tail;
+    static const bool init = [] {
+        for(int k = 0; k < kAcosTerms; ++k) { clo[k] = acos_coeff(k, false); chi[k] = acos_coeff(k, true); }
+        double r = acos_coeff(kAcosTerms, true);
+        for(int i = 0; i < kAcosTerms; ++i) r *= 0.25;
+        tail = r * 2.0;
+        return true;
+    }();
+    (void)init;
+    if(!(t >= -1.0 && t <= 1.0)) { lo = hi = std::numeric_limits<double>::quiet_NaN(); return; }
+    const double a = t < 0.0 ? -t : t;
+    const bool small = a <= 0.5;
+    const double u = opq(-opq(a - 1.0) * 0.5);   // exact; +0 for a = 1 (a - 1 is -0 rounding down)
+    const double zl = small ? rd_mul(a, a) : u, zh = small ? ru_mul(a, a) : u;
+    double pl = clo[kAcosTerms - 1], ph = chi[kAcosTerms - 1];
+    for(int k = kAcosTerms - 2; k >= 0; --k) {
+        pl = rd_fma(zl, pl, clo[k]);
+        ph = ru_fma(zh, ph, chi[k]);
+    }
+    ph = ru_add(ph, tail);
+    double ml = a, mh = a;
+    if(!small) { ml = 2.0 * rd_sqrt(u); mh = 2.0 * ru_sqrt(u); }
+    const double rl = rd_mul(ml, pl), rh = ru_mul(mh, ph);
+    if(small) {
+        lo = t >= 0.0 ? rd_sub(kHalfPiLo, rh) : rd_add(kHalfPiLo, rl);
+        hi = t >= 0.0 ? ru_sub(kHalfPiHi, rl) : ru_add(kHalfPiHi, rh);
+    } else {
+        lo = t >= 0.0 ? rl : rd_sub(kPiLo, rh);
+        hi = t >= 0.0 ? rh : ru_sub(kPiHi, rl);
+    }
+}
 IVT<float> interval_acos(IVT<float> x, CT B) {
-    const IV r = interval_acos(IV{x.lb, x.ub, x.und}, B);
-    return IVT<float>{f32_down(r.lb), f32_up(r.ub), r.und};
+    constexpr std::int64_t kOne = 1000000;
+    bool und = x.und;
+    double l = x.lb, u = x.ub;
+    if(B.lb < -kOne && l < -1.0) { l = -1.0; und = true; }
+    if(B.ub > kOne && u > 1.0) { u = 1.0; und = true; }
+    double lo, hi, dummy;
+    acos_enclose(u, lo, dummy);
+    acos_enclose(l, dummy, hi);
+    return IVT<float>{f32_down(lo), f32_up(hi), und};
 }
 
 template<class X> inline X dmax(X n1, X n2) { return n2 < n1 ? n1 : n2; }
@@ -1097,7 +1160,7 @@ int oracle_rounded_op(int op, std::uint64_t n, const double* a, const double* b,
                 }
                 case 13: {
                     const float x = static_cast<float>(a[i]);
-                    const IVT<float> r = interval_acos(IVT<float>{x, x, false}, kUnbounded);
+                    const IVT<float> r = interval_acos(IVT<float>{x, x, false}, CT{-1000000, 1000000});   // no clamping
                     lo[i] = r.lb;
                     hi[i] = r.ub;
                     break;

@@ -22,7 +22,7 @@ __global__ void k_rounded_op(int op, unsigned long long n, const double* a, cons
             case 4: l = rd_sqrt(x); h = ru_sqrt(x); break;
             case 5: { bool hard; acosfx::acos_directed(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
             case 6: { bool hard; acosfx::acos_series(x, &l, &h, &hard); if(hard) atomicAdd(&g_acos_hard, 1ull); break; }
-            case 13: {   // binary32 acos: outward conversion of the binary64 directed results
+            case 13: {   // binary32 acos: the outward-converted enclosure (no domain clamping)
                 const IVT<float> r = iv_acos<false, false>(IVT<float>{static_cast<float>(x), static_cast<float>(x), false});
                 l = r.lb; h = r.ub;
                 break;

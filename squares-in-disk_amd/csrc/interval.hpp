@@ -16,6 +16,7 @@
 #pragma once
 #include "rounding.hpp"
 #include "acos_fixed.hpp"
+#include "acos_enclosure.hpp"
 
 namespace sqd {
 namespace dev {
@@ -207,12 +208,20 @@ __device__ inline IV iv_acos(IV x) {
     if(hard1 || hard2) atomicAdd(&g_acos_hard, 1ull);
     return IV{l1, h2, und};
 }
-// The binary32 version: the binary64 directed results converted outward (the
-// largest float below RD64(acos x) is the largest float below acos x).
+// The binary32 version: the same domain handling, and the binary64 enclosure of
+// acosenc::acos_enclose (about 2^-39 wide) converted outward, which contains
+// acos and is the correctly rounded pair of floats but in rare cases (then one
+// float wider); the binary32 pass only has to enclose.
 template<bool CheckLb, bool CheckUb>
 __device__ inline IVT<float> iv_acos(IVT<float> x) {
-    const IV r = iv_acos<CheckLb, CheckUb>(IV{static_cast<double>(x.lb), static_cast<double>(x.ub), x.und});
-    return IVT<float>{f32_down(r.lb), f32_up(r.ub), r.und};
+    bool und = x.und;
+    double l = static_cast<double>(x.lb), u = static_cast<double>(x.ub);
+    if(CheckLb && l < -1.0) { l = -1.0; und = true; }
+    if(CheckUb && u > 1.0) { u = 1.0; und = true; }
+    double lo, hi, dummy;
+    acosenc::acos_enclose(u, lo, dummy);
+    acosenc::acos_enclose(l, dummy, hi);
+    return IVT<float>{f32_down(lo), f32_up(hi), und};
 }
 
 template<class X> __device__ __forceinline__ X dmax(X n1, X n2) { return n2 < n1 ? n1 : n2; }

@@ -280,6 +280,10 @@ struct sqd_service {
     std::map<const sqd_program*, uint64_t> core_bytes;     // measured size of a core per program
     std::map<const sqd_program*, sqd_settings> settings;   // per program (autotuned or the defaults)
     std::map<const sqd_program*, bool> tuning;             // autotune in progress
+    // continuous batches running per program: their program's queued jobs join
+    // them as rows free up, so another worker starts a further batch (and a
+    // further core) of the program only when a whole batch of its jobs waits
+    std::map<const sqd_program*, int> running_progs;
     bool stop = false;
     std::vector<std::thread> workers;
 
@@ -373,9 +377,30 @@ struct sqd_service {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(m);
         for(;;) {
-            cv.wait(lk, [&] { return stop || !queue.empty(); });
+            // the program of the oldest queued job that no batch runs, or that has
+            // at least a batch of jobs queued
+            const sqd_program* prog = nullptr;
+            cv.wait(lk, [&] {
+                if(stop) return true;
+                std::map<const sqd_program*, uint32_t> waiting;
+                for(uint64_t id : queue) ++waiting[jobs[id].prog];
+                for(uint64_t id : queue) {
+                    const sqd_program* p = jobs[id].prog;
+                    auto r = running_progs.find(p);
+                    if(r == running_progs.end() || r->second == 0 || waiting[p] >= opt.batch_jobs) { prog = p; return true; }
+                }
+                return false;
+            });
             if(stop) return;
-            const sqd_program* prog = jobs[queue.front()].prog;
+            ++running_progs[prog];
+            struct Release {                   // every exit below holds the lock
+                sqd_service* s;
+                const sqd_program* p;
+                ~Release() {
+                    if(--s->running_progs[p] == 0) s->running_progs.erase(p);
+                    s->cv.notify_all();
+                }
+            } release{this, prog};
             // settings of the program (tuned once, by the first worker that needs them)
             if(!settings.count(prog)) {
                 if(opt.autotune && !tuning[prog]) {

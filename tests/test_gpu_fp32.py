@@ -33,8 +33,14 @@ def test_binary32_ops_bit_exact(op):
     b = _floats(6, 200000)
     if op == "sqrt32":
         a = np.abs(a)
-    if op == "acos32":
-        a = np.random.default_rng(7).uniform(-1, 1, 100000).astype(np.float32).astype(np.float64)
+    if op == "acos32":                                # the prefilter's enclosure (tests/test_acos_enclosure.py)
+        rng = np.random.default_rng(7)
+        a = np.concatenate([rng.uniform(-1, 1, 100000), 1 - np.exp2(-rng.uniform(1, 24, 20000)),
+                            -1 + np.exp2(-rng.uniform(1, 24, 20000)),
+                            np.exp2(-rng.uniform(1, 126, 10000)) * rng.choice([-1, 1], 10000),
+                            0.5 + rng.uniform(-1e-3, 1e-3, 5000), -0.5 + rng.uniform(-1e-3, 1e-3, 5000),
+                            [1.0, -1.0, 0.0, -0.0, 0.5, -0.5, 1.5, -2.0]])
+        a = a.astype(np.float32).astype(np.float64)
         b = a
     if op == "to32":
         a = np.random.default_rng(8).standard_normal(200000) * np.exp2(np.random.default_rng(9).integers(-140, 130, 200000))
