@@ -223,34 +223,116 @@ __host__ __device__ inline int cos_ge(double y, double x) {
     return -1;
 }
 
+// A cheaper first try of cos_ge, decisive in all but rare cases: the same
+// fixed point with two savings.
+//  * mulq_low drops the limb products below 2^96 (with 32-bit limbs a_i, b_j,
+//    those with i + j <= 2): they add less than 2^64 (1 + 2 2^32 + 3 2^64) <
+//    3.01 2^128 before the shift by 124, so the result is at most 49 units
+//    below floor(a b / 2^124) and never above it.  10 of the 16 limb products.
+//  * 12 terms of each series: the alternating tails are at most
+//    z^12 / 24! < 2^36.6 units (cos) and a z^12 / 25! < 2^31.6 units (sin) for
+//    z < 0.617, a < 0.786.
+// Rounding errors (units of 2^-124): a product is less than 50 below the exact
+// one (49 dropped, 1 floor) and z less than 50 below z; each Horner step adds
+// less than 1 (coefficient) + 50 (product) + 50 (z's error times a partial sum
+// <= 1) and scales the earlier error by z < 0.617, so the sum is within
+// 101 / 0.383 < 264 units, and sin's final product keeps it below 50 + 0.786 x
+// 264 < 264.  With the tails, pi/2 and x (under 3 units) the evaluation is
+// within 2^37 units; kCosErrFast allows twice that.  The cosines of neighbouring
+// doubles y differ by about 2^-53 sin y, so away from y = 0 and pi the test is
+// undecided in about one comparison in 2^32 (cos_ge then decides); near them
+// (x near +-1) cos_ge decides more often.
+constexpr u128 kCosErrFast = kUnit << 38;
+constexpr int kCosTermsFast = 12;
+
+// floor(a b / 2^124) minus 0 to 49, for a, b < 2^125 (limb products of weight
+// >= 2^96 only; the sum of weight 2^128 and above fits 128 bits).
+__host__ __device__ inline u128 mulq_low(u128 a, u128 b) {
+    std::uint32_t x[4], w[4];
+    for(int i = 0; i < 4; ++i) {
+        x[i] = static_cast<std::uint32_t>(a >> (32 * i));
+        w[i] = static_cast<std::uint32_t>(b >> (32 * i));
+    }
+    auto p = [&](int i, int j) { return u128(static_cast<std::uint64_t>(x[i]) * w[j]); };
+    const u128 lo = p(0, 3) + p(1, 2) + p(2, 1) + p(3, 0);                       // weight 2^96
+    const u128 hi = p(1, 3) + p(2, 2) + p(3, 1) + ((p(2, 3) + p(3, 2)) << 32) + (p(3, 3) << 64);   // weight 2^128
+    return (hi << 4) + (lo >> 28);
+}
+__host__ __device__ inline u128 sin_fx_fast(u128 a) {
+    constexpr TrigCoeffs T = trig_coeffs();
+    const u128 z = mulq_low(a, a);
+    u128 s = T.s[kCosTermsFast - 1];
+    for(int k = kCosTermsFast - 2; k >= 0; --k) s = T.s[k] - mulq_low(z, s);
+    return mulq_low(a, s);
+}
+__host__ __device__ inline u128 cos_fx_fast(u128 b) {
+    constexpr TrigCoeffs T = trig_coeffs();
+    const u128 z = mulq_low(b, b);
+    u128 s = T.c[kCosTermsFast - 1];
+    for(int k = kCosTermsFast - 2; k >= 0; --k) s = T.c[k] - mulq_low(z, s);
+    return s;
+}
+// cos_ge at the cheaper precision; -1 also when that does not decide.
+__host__ __device__ inline int cos_ge_fast(double y, double x) {
+    const u128 Y = static_cast<u128>(to_fx_signed(y));
+    const u128 H = half_pi_fx();
+    const bool neg = Y >= H;
+    const u128 r = neg ? Y - H : H - Y;
+    const u128 Q = H >> 1;
+    const u128 s = r <= Q ? sin_fx_fast(r) : cos_fx_fast(r <= H ? H - r : r - H);
+    const i128 c = neg ? -static_cast<i128>(s) : static_cast<i128>(s);
+    const i128 X = to_fx_signed(x);
+    const i128 e = static_cast<i128>(kCosErrFast);
+    if(c - e > X) return 1;
+    if(c + e < X) return 0;
+    return -1;
+}
+// cos_ge at full precision, out of line: the walk needs it rarely (see
+// cos_ge_fast), and inline its 128-bit temporaries would raise the register
+// count, and so lower the occupancy, of every kernel that evaluates acos.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __noinline__ int cos_ge_full(double y, double x) { return cos_ge(y, x); }
+#else
+inline int cos_ge_full(double y, double x) { return cos_ge(y, x); }
+#endif
+// cos_ge, tried at the cheaper precision first where that usually decides
+// (y at least 2^-10 away from 0 and pi: neighbouring cosines at least about
+// 2^-63 apart).
+__host__ __device__ inline int cos_ge_2(double y, double x) {
+    if(y >= 0x1p-10 && y <= 0x1.91ffb54442d18p+1) {       // pi - 2^-10, rounded down
+        const int t = cos_ge_fast(y, x);
+        if(t >= 0) return t;
+    }
+    return cos_ge_full(y, x);
+}
+
 // RD(acos(x)) for x in (-1, 1) by walking from an approximation y0 with cos_ge;
 // false when the walk does not settle within a few steps or a comparison is
-// undecided (the caller then uses the series below).
+// undecided (the caller then uses the series below).  One comparison per step,
+// at one place in the code: from y0 upward while cos(y) >= x (the last such y
+// is the result), or downward until cos(y) >= x.
 __host__ __device__ inline bool acos_walk(double x, double y0, double* lo) {
     constexpr std::uint64_t kPiDown = 0x400921fb54442d18ull;   // largest double below pi
     const double pi_down = from_bits_h(kPiDown);
     double y = y0;
     if(!(y >= 0x1p-60)) y = 0x1p-60;                 // also catches NaN
     if(y > pi_down) y = pi_down;
-    int t = cos_ge(y, x);
-    if(t < 0) return false;
-    if(t == 1) {                                      // y <= acos(x): step up while cos stays >= x
-        for(int i = 0; i < 8; ++i) {
+    int dir = 0;                                      // +1 up (cos(y0) >= x), -1 down
+    double prev = y;
+    for(int i = 0; i < 9; ++i) {
+        const int t = cos_ge_2(y, x);
+        if(t < 0) return false;
+        if(i == 0) dir = t == 1 ? 1 : -1;
+        if(dir > 0) {
+            if(t == 0) { *lo = prev; return true; }
             if(y == pi_down) { *lo = y; return true; }
-            const double yn = from_bits_h(to_bits_h(y) + 1);
-            const int tn = cos_ge(yn, x);
-            if(tn < 0) return false;
-            if(tn == 0) { *lo = y; return true; }
-            y = yn;
+            prev = y;
+            y = from_bits_h(to_bits_h(y) + 1);
+        } else {
+            if(t == 1) { *lo = y; return true; }
+            y = from_bits_h(to_bits_h(y) - 1);
+            if(!(y >= 0x1p-60)) return false;
         }
-        return false;
-    }
-    for(int i = 0; i < 8; ++i) {                      // y > acos(x): step down until cos(y) >= x
-        y = from_bits_h(to_bits_h(y) - 1);
-        if(!(y >= 0x1p-60)) return false;
-        const int tn = cos_ge(y, x);
-        if(tn < 0) return false;
-        if(tn == 1) { *lo = y; return true; }
     }
     return false;
 }
@@ -305,6 +387,13 @@ __host__ __device__ inline void acos_series(double x, double* lo, double* hi, bo
     *hard = (dl != dh) || (ul != uh);
 }
 
+// acos_series out of line (the rare fallback of acos_directed).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __noinline__ void acos_series_call(double x, double* lo, double* hi, bool* hard) { acos_series(x, lo, hi, hard); }
+#else
+inline void acos_series_call(double x, double* lo, double* hi, bool* hard) { acos_series(x, lo, hi, hard); }
+#endif
+
 // RD(acos(x)) and RU(acos(x)) for x in [-1, 1]: the cos walk from the math
 // library's acos (any approximation works: the walk decides), else the series.
 // acos(x) is irrational for x != 1, so RU is the next double above RD.
@@ -318,7 +407,7 @@ __host__ __device__ inline void acos_directed(double x, double* lo, double* hi, 
             return;
         }
     }
-    acos_series(x, lo, hi, hard);
+    acos_series_call(x, lo, hi, hard);
 }
 
 } // namespace acosfx

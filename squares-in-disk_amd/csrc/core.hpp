@@ -26,6 +26,7 @@
 //   deps(list, f)            calls f(a) for each bound index a in update list `list`
 //   rows_violated<L>(lb, ub, C)   constraint rows of static level L
 //   all_satisfied(lb, ub, C)      every remaining constraint possibly holds
+//   level_waves(L), prop_fast(L)  per-level register budget and propagation form
 #pragma once
 #include <type_traits>
 #include "interval.hpp"
@@ -182,34 +183,38 @@ __device__ __forceinline__ void rec_inc(unsigned long long (&rec)[3], int a) {
     if(a < 16) rec[0] += inc; else if(a < 32) rec[1] += inc; else rec[2] += inc;
 }
 
-template<class P, class Num>
+// FAST (the codegen's P::prop_fast for a level; the results are the same
+// either way): the queue starts with the bounds that depend on `arg` and is
+// worked off in order.  Every call site passes a compile-time `arg`, so the
+// initial bounds are known here: they are applied first, in queue order,
+// without the queue.  While none of them changes the box nothing else is
+// queued, and when all leave it unchanged the propagation is over (the common
+// case).  From the first change on, the queue below takes over in exactly the
+// state the plain algorithm reaches there: the initial bounds after it still
+// queued, and the dependents of its change queued after them.  The inlined
+// initial bounds cost registers, which lowers the occupancy of some kernels.
+template<class P, bool FAST = true, class Num>
 __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int arg, int it_limit, unsigned rec_limit,
                           const Num* C, PropScratch<P>& s)
 {
     constexpr int NB = P::NB;
     static_assert(NB <= 48, "at most 48 bounds (4-bit recursion counters in three words)");
     if(NB == 0 || it_limit <= 0) return false;
-    // The queue starts with the bounds that depend on `arg` and is worked off
-    // in order.  Every call site passes a compile-time `arg`, so the initial
-    // bounds are known here: they are applied first, in queue order, without
-    // the queue.  While none of them changes the box nothing else is queued, and
-    // when all leave it unchanged the propagation is over (the common case).
-    // From the first change on, the queue below takes over in exactly the state
-    // the plain algorithm reaches there: the initial bounds after it still
-    // queued, and the dependents of its change queued after them.
-    // A queue that receives all NB bounds at once reads as empty (begin == end,
-    // see the ring below), and then nothing is applied.
-    int m = 0;
-    P::deps(arg * 3 + 2, [&](int) { ++m; });
-    if(m >= NB) return false;
     int done = 0, r_first = 0;
-    P::deps(arg * 3 + 2, [&](int k) {
-        if(r_first != 0) return;
-        r_first = P::apply(k, lb, ub, C);
-        ++done;
-    });
-    if(r_first < 0) return true;
-    if(r_first == 0) return false;
+    if constexpr(FAST) {
+        // a queue that receives all NB bounds at once reads as empty (begin ==
+        // end, see the ring below), and then nothing is applied
+        int m = 0;
+        P::deps(arg * 3 + 2, [&](int) { ++m; });
+        if(m >= NB) return false;
+        P::deps(arg * 3 + 2, [&](int k) {
+            if(r_first != 0) return;
+            r_first = P::apply(k, lb, ub, C);
+            ++done;
+        });
+        if(r_first < 0) return true;
+        if(r_first == 0) return false;
+    }
     const int lane = threadIdx.x;
     int qb = 0, qe = 0;
     unsigned long long present = 0, unstable = 0;
@@ -240,7 +245,7 @@ __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int 
     P::deps(arg * 3 + 2, [&](int a) {
         if(seen++ >= done) enqueue(a);   // the initial bounds not yet applied
     });
-    handle(r_first);
+    if(FAST) handle(r_first);
     for(int it = 0; it < it_limit; ++it) {
         while(qb != qe) {
             const int k = s.ring[qb][lane];
@@ -480,7 +485,7 @@ __device__ __forceinline__ void static_decide(const LevelArgs& args, unsigned lo
     unsigned origin = 0, leaf = 0;
     if(valid) {
         static_child<P, L>(args, in_base, w, lb, ub, origin, leaf);
-        if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+        if(propagate<P, P::prop_fast(L)>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
             ++c_leaf;
         } else if(P::template rows_violated<L>(lb, ub, s_c)) {
             ++c_leaf;
@@ -533,7 +538,7 @@ __device__ __forceinline__ void static_decide_hoisted(const LevelArgs& args, boo
     if(valid) {
         lb[ARG] = split_point(pl, pu, wd, i, SUBS);
         ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
-        if(propagate<P>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+        if(propagate<P, P::prop_fast(L)>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
             ++c_leaf;
         } else {
             bool same = true;
@@ -798,7 +803,7 @@ __global__ __launch_bounds__(kBlock) SQD_LEVEL_ATTR void k_static_prefilter(Leve
                 lb32[ARG] = f32_down(split_point(lb[ARG], ub[ARG], wd, i, SUBS));
                 ub32[ARG] = f32_up(split_point(lb[ARG], ub[ARG], wd, i + 1, SUBS));
                 ++c_chk;
-                bool pruned = propagate<P>(lb32, ub32, ARG, N, 4u, s_c32, s_prop);
+                bool pruned = propagate<P, P::prop_fast(L)>(lb32, ub32, ARG, N, 4u, s_c32, s_prop);
                 if(!pruned) {
                     if constexpr(SHARE) {
                         plb[ARG] = lb32[ARG];

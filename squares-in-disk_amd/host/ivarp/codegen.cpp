@@ -522,12 +522,23 @@ std::string generate(const flat::Program& p, const std::string& key) {
     // Register budget of the binary64 static level kernels as resident waves
     // per SIMD (0: the compiler's choice).  Measured per kernel: a budget of 4
     // (128 VGPRs, a few spilled) made One's and Two's level 1 and More2's
-    // level 2 faster and Four's level 2 slower.
+    // level 2 faster and Four's level 2 slower; 3 (168 VGPRs, no spills) made
+    // More1's level 1 faster than the compiler's 176 VGPRs at 2 waves (301 ->
+    // 264 ms per proof).
     int waves_l1 = 0, waves_l2 = 0;
     if(key == "one" || key == "two") waves_l1 = 4;
+    if(key == "more1") waves_l1 = 3;
     if(key == "more2") waves_l2 = 4;
     o << "    static constexpr int level_waves(int l) { return l == 1 ? " << waves_l1 << " : l == 2 ? " << waves_l2
       << " : 0; }\n";
+    // Whether the static levels' propagation applies the initial bounds inline
+    // before it builds its queue (same results; fewer instructions, more
+    // registers).  Measured per kernel: faster for every statement's dominant
+    // level but Four's level 2 (193 instead of 171 VGPRs: 2 waves per SIMD
+    // instead of 3, 4.72 against 4.37 ms per proof).
+    int slow_prop_level = -1;
+    if(key == "four") slow_prop_level = 2;
+    o << "    static constexpr bool prop_fast(int l) { return l != " << slow_prop_level << "; }\n";
     o << body.str();
     o << "};\n\n";
     o << "static const double k_consts[] = {";
