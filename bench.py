@@ -96,6 +96,7 @@ WORKLOADS = {
     "two": "two_subcontainer_plain_fp64",
     "three": "three_subcontainer_plain_fp64",
     "four": "four_subcontainer_plain_fp64",
+    "more1": "more1_plain_fp64",
 }
 MODES = {0: "NONFINAL", 1: "FINAL", 2: "SPLIT", 3: "INIT"}
 
@@ -123,6 +124,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the Four and More1 legs")
+    ap.add_argument("--legs", default="four,more1", help="which of the Four and More1 legs to run")
     ap.add_argument("--no-decomposition", action="store_true", help="skip the batched-ratio decomposition")
     ap.add_argument("--leg-proofs", type=int, default=64, help="batched Four proofs per configuration")
     ap.add_argument("--cpu-sample-cuboids", type=float, default=6e6,
@@ -513,10 +515,29 @@ def lemma_leg(sqd, key, plain_steps, sample_cuboids, workers, batch_jobs, proofs
             "value": timed["num_cuboids"] / timed["seconds"], "unit": "cuboids/s", "cores": threads, "kind": "port",
             "sample": "initial subdivisions [%d, %d) of the first variable: %d cuboids (equal to the device's), "
                       "%.2f s with %d threads" % (a, b, timed["num_cuboids"], timed["seconds"], threads)}
-        out["roofline"] = leg_roofline(dom, counted["phases"])
-        out["roofline_fp32_first"] = leg_roofline(dom32, counted32["phases"], fp32=bool(best.fp32_prefilter))
+        out["roofline"] = attach_counters(leg_roofline(dom, counted["phases"]), key)
+        out["roofline_fp32_first"] = attach_counters(leg_roofline(dom32, counted32["phases"],
+                                                                  fp32=bool(best.fp32_prefilter)), key)
         out["op_count_sample"] = "the same range, op-counting restatement"
     return out
+
+
+def attach_counters(roof, key):
+    """HBM traffic and the VALU-issue bound of a leg's dominant kernel from its
+    counter summary under profiles/ (traffic_<kernel kind>_<statement>.txt), when
+    one covers that kernel."""
+    if not roof:
+        return roof
+    kind = roof["kernel"].split("<")[0]
+    path = ROOT / "profiles" / ("traffic_%s_%s.txt" % (kind, key))
+    traffic = read_traffic(path, roof["kernel"])
+    issue_ms = valu_issue_bound_ms(path, roof["kernel"])
+    if traffic is not None:
+        roof["traffic"] = traffic
+    if issue_ms is not None:
+        roof["valu_issue_bound_ms"] = issue_ms
+        roof["valu_issue_frac"] = issue_ms / roof["mean_launch_ms"]
+    return roof
 
 
 def width_sample(sqd, prog, n_boxes=8192, seed=3):
@@ -649,12 +670,14 @@ def main():
         decomposition[a.lemma] = decompose(sqd, a.lemma, a.workers, a.batch_jobs, a.batched_proofs, fp32_first=mine)
         batched["ratio_vs_plain_batched"] = decomposition[a.lemma]["ratio_vs_plain_batched"]
     legs = {}
-    if not a.no_legs:
+    which = set() if a.no_legs else set(a.legs.split(","))
+    if "four" in which:
         legs["four_fp32_first"] = lemma_leg(sqd, "four", 10, a.cpu_sample_cuboids, a.workers, a.batch_jobs,
                                             a.leg_proofs, cpu=not a.no_cpu_baseline,
                                             batched=not a.no_decomposition)
         if "batched" in legs["four_fp32_first"]:
             decomposition["four"] = legs["four_fp32_first"]["batched"]
+    if "more1" in which:
         legs["more1_fp32_first"] = lemma_leg(sqd, "more1", 3, a.cpu_sample_cuboids, a.workers, a.batch_jobs,
                                              0, cpu=not a.no_cpu_baseline, batched=False)
     stream = None if a.no_stream else job_stream(sqd, a.stream_jobs, a.workers, a.stream_batch_jobs, a.seed)

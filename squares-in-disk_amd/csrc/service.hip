@@ -378,7 +378,9 @@ struct sqd_service {
         std::unique_lock<std::mutex> lk(m);
         for(;;) {
             // the program of the oldest queued job that no batch runs, or that has
-            // at least a batch of jobs queued
+            // at least a batch of jobs queued and an idle core (a new core costs
+            // its allocation and first launches: only for a backlog of four
+            // batches)
             const sqd_program* prog = nullptr;
             cv.wait(lk, [&] {
                 if(stop) return true;
@@ -387,7 +389,11 @@ struct sqd_service {
                 for(uint64_t id : queue) {
                     const sqd_program* p = jobs[id].prog;
                     auto r = running_progs.find(p);
-                    if(r == running_progs.end() || r->second == 0 || waiting[p] >= opt.batch_jobs) { prog = p; return true; }
+                    if(r == running_progs.end() || r->second == 0) { prog = p; return true; }
+                    if(waiting[p] < opt.batch_jobs) continue;
+                    bool idle = false;
+                    for(const Slot& sl : pool) idle = idle || (sl.prog == p && !sl.busy && sl.core);
+                    if(idle || waiting[p] >= 4 * opt.batch_jobs) { prog = p; return true; }
                 }
                 return false;
             });

@@ -286,7 +286,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
     std::map<int, bool> roots;
     for(const auto& r : p.rows) for(int c : r) roots[c] = true;
     for(const auto& kv : roots) {
-        body << "    template<class Num> __device__ static IB c" << kv.first
+        body << "    template<class Num> __device__ __forceinline__ static IB c" << kv.first
              << "(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
         body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n";
         body << e.eval(kv.first, "        ");
@@ -333,7 +333,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
     for(std::size_t j = 0; j < cons.size(); ++j) body << "        case " << j << ": return c" << cons[j] << "(lb, ub, C);\n";
     body << "        default: return IB{false, true};\n        }\n    }\n";
     // bound application
-    body << "    template<class Num> __device__ static int apply(int k, Num (&lb)[N], Num (&ub)[N], const Num* C) {\n";
+    body << "    template<class Num> __device__ __forceinline__ static int apply(int k, Num (&lb)[N], Num (&ub)[N], const Num* C) {\n";
     body << "        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C;\n        switch(k) {\n";
     for(std::size_t k = 0; k < p.bounds.size(); ++k) {
         const flat::FBound& bd = p.bounds[k];
@@ -378,7 +378,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
     }
     body << "        default: break;\n        }\n    }\n";
     // constraint rows per static level, tested in row order (cpu_prover_core.hpp:179-191)
-    body << "    template<int L, class Num> __device__ static bool rows_violated(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+    body << "    template<int L, class Num> __device__ __forceinline__ static bool rows_violated(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
     body << "        (void)lb; (void)ub; (void)C;\n";
     for(std::size_t L = 0; L < sta.size(); ++L) {
         int from = p.splits[static_cast<std::size_t>(sta[L])].arg;
@@ -391,7 +391,7 @@ std::string generate(const flat::Program& p, const std::string& key) {
         body << "        }\n";
     }
     body << "        return false;\n    }\n";
-    body << "    template<class Num> __device__ static bool all_satisfied(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
+    body << "    template<class Num> __device__ __forceinline__ static bool all_satisfied(const Num (&lb)[N], const Num (&ub)[N], const Num* C) {\n";
     for(int c : p.rows[static_cast<std::size_t>(p.nargs - 1)]) body << "        if(!c" << c << "(lb, ub, C).poss) return false;\n";
     body << "        (void)lb; (void)ub; (void)C;\n        return true;\n    }\n";
 
@@ -403,10 +403,10 @@ std::string generate(const flat::Program& p, const std::string& key) {
     // rows_violated_hoisted<L> evaluates the rest per child, with the same
     // results as rows_violated<L> on that box.
     std::ostringstream hoist_types, hoist_fns;
-    hoist_fns << "    template<int L, class Num> __device__ static void hoist(const Num (&lb)[N], const Num (&ub)[N], "
+    hoist_fns << "    template<int L, class Num> __device__ __forceinline__ static void hoist(const Num (&lb)[N], const Num (&ub)[N], "
                  "const Num* C, Hoist<L, Num>& h) {\n        using IV = IVT<Num>;\n        (void)lb; (void)ub; (void)C; (void)h;\n";
     std::ostringstream rows_h;
-    rows_h << "    template<int L, class Num> __device__ static bool rows_violated_hoisted(const Num (&lb)[N], "
+    rows_h << "    template<int L, class Num> __device__ __forceinline__ static bool rows_violated_hoisted(const Num (&lb)[N], "
               "const Num (&ub)[N], const Num* C, const Hoist<L, Num>& h) {\n        using IV = IVT<Num>;\n"
               "        (void)lb; (void)ub; (void)C; (void)h;\n";
     std::string hoist_sel = "NoHoist";
@@ -520,15 +520,16 @@ std::string generate(const flat::Program& p, const std::string& key) {
     table("st_arg", sta, [](const flat::FSplit& s) { return s.arg; });
     table("st_subs", sta, [](const flat::FSplit& s) { return static_cast<int>(s.subs); });
     // Register budget of the binary64 static level kernels as resident waves
-    // per SIMD (0: the compiler's choice).  Measured per kernel: a budget of 4
-    // (128 VGPRs, a few spilled) made One's and Two's level 1 and More2's
-    // level 2 faster and Four's level 2 slower; 3 (168 VGPRs, no spills) made
-    // More1's level 1 faster than the compiler's 176 VGPRs at 2 waves (301 ->
-    // 264 ms per proof).
+    // per SIMD (0: the compiler's choice), measured per kernel (per proof, with
+    // the constraint functions inlined): One's and Two's level 1 at 4 (128
+    // VGPRs, a few spilled; 3 was slower), Three's level 1 at 4 (274 against
+    // 292 us), More1's level 1 at 3 (264 against 301 ms at the compiler's 176
+    // VGPRs), Four's level 2 at 3 (3.46 against 3.92 ms) and More2's level 2 at
+    // 3 (19.8 ms; 4 spilled 426 VGPRs and took 38.7 ms).
     int waves_l1 = 0, waves_l2 = 0;
-    if(key == "one" || key == "two") waves_l1 = 4;
+    if(key == "one" || key == "two" || key == "three") waves_l1 = 4;
     if(key == "more1") waves_l1 = 3;
-    if(key == "more2") waves_l2 = 4;
+    if(key == "four" || key == "more2") waves_l2 = 3;
     o << "    static constexpr int level_waves(int l) { return l == 1 ? " << waves_l1 << " : l == 2 ? " << waves_l2
       << " : 0; }\n";
     // Whether the static levels' propagation applies the initial bounds inline
