@@ -125,6 +125,9 @@ def parse():
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the Four and More1 legs")
     ap.add_argument("--legs", default="four,more1", help="which of the Four and More1 legs to run")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="cores launch every round's kernels directly instead of replaying HIP graphs "
+                         "(the same kernels; for counter collection)")
     ap.add_argument("--no-decomposition", action="store_true", help="skip the batched-ratio decomposition")
     ap.add_argument("--leg-proofs", type=int, default=64, help="batched Four proofs per configuration")
     ap.add_argument("--cpu-sample-cuboids", type=float, default=6e6,
@@ -621,6 +624,14 @@ def main():
               "part of this build" % (a.gpus, world), file=sys.stderr)
         sys.exit(2)
     import sqd
+    if a.no_graphs:                      # every core this run makes, through the binding's defaults
+        plain_defaults = sqd.default_settings
+
+        def no_graph_defaults(**kw):
+            st = plain_defaults(**kw)
+            st.graphs = 0
+            return st
+        sqd.default_settings = no_graph_defaults
 
     prog = sqd.Program.from_lemma(a.lemma)
     expected = PUBLISHED[a.lemma]
