@@ -195,6 +195,9 @@ int sqd_prove_batch(sqd_core* core, size_t n_jobs, const uint32_t* first, const 
  * continuous batch the same way. */
 int sqd_batch_admit(sqd_core* core, uint32_t row, uint32_t first, uint32_t last);
 int sqd_batch_step(sqd_core* core, sqd_job_counts* results, uint64_t* finished, uint32_t* active);
+/* The cuboids on each row's frontier after the last step (counts[j] for rows
+ * j < n and j < sqd_settings.jobs; rows without a job hold 0). */
+int sqd_batch_frontier(sqd_core* core, uint64_t* counts, size_t n);
 
 /* The critical cuboids of job `job` of the last sqd_prove_batch (as
  * sqd_core_criticals: up to capacity boxes, SoA stride capacity; *n the total). */
@@ -357,6 +360,8 @@ typedef struct sqd_service_options {
     uint32_t batch_jobs;          /* queued jobs of one program a worker proves together
                                      (sqd_prove_batch; each core holds that many; default 8) */
     uint32_t reserved;
+    uint64_t frontier_budget;     /* bytes of frontier the running jobs may hold together
+                                     (0: no limit, the default); see sqd_service_get_stats */
 } sqd_service_options;
 typedef struct sqd_job_result {
     sqd_counts counts;
@@ -384,6 +389,25 @@ int sqd_service_poll(sqd_service* service, uint64_t job, int* done, sqd_job_resu
 int sqd_service_wait(sqd_service* service, uint64_t job, sqd_job_result* out);
 /* Stops the workers (running jobs finish, queued ones are dropped) and frees the cores. */
 void sqd_service_destroy(sqd_service* service);
+
+/* Admission by frontier bytes.  A running job's frontier (the cuboids on its
+ * device stack, 16 bytes per variable plus 4 each) is counted after every
+ * round; a queued job is admitted (into a free row of a running batch, or a new
+ * batch) only while the running jobs' frontier bytes plus its own estimate stay
+ * within frontier_budget, or when no job runs at all.  A job's estimate is its
+ * program's largest frontier peak per initial subdivision among the finished
+ * jobs (at least one cuboid), times its range.  No counterpart in the
+ * reference; for callers sharing one device among many jobs. */
+typedef struct sqd_service_stats {
+    uint64_t frontier_bytes;      /* running jobs' frontier bytes now */
+    uint64_t frontier_bytes_peak; /* most of them at once */
+    uint32_t rows_running;        /* jobs running now (rows of continuous batches) */
+    uint32_t rows_peak;           /* most at once */
+    uint64_t cores_created;
+    uint64_t cores_destroyed;     /* idle cores destroyed to make room */
+    uint64_t jobs_completed;
+} sqd_service_stats;
+int sqd_service_get_stats(sqd_service* service, sqd_service_stats* out);
 
 /* ---------------------------------------------------------------- timing */
 

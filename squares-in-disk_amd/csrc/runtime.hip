@@ -1547,6 +1547,16 @@ int sqd_batch_step(sqd_core* c, sqd_job_counts* results, uint64_t* finished, uin
     });
 }
 
+int sqd_batch_frontier(sqd_core* c, uint64_t* counts, size_t n) {
+    return guarded([&] {
+        if(!c || (!counts && n)) throw std::invalid_argument("null argument");
+        for(size_t j = 0; j < n; ++j) {
+            const bool held = j < c->row_state.size() && c->row_state[j] != 0 && j < c->states;
+            counts[j] = held ? c->h_state[j].stack_count : 0ull;
+        }
+    });
+}
+
 int sqd_core_job_criticals(sqd_core* c, size_t job, double* crit_lb, double* crit_ub, size_t capacity, size_t* n) {
     return guarded([&] {
         if(!c || !n) throw std::invalid_argument("null argument");

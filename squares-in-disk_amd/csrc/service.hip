@@ -284,6 +284,13 @@ struct sqd_service {
     // them as rows free up, so another worker starts a further batch (and a
     // further core) of the program only when a whole batch of its jobs waits
     std::map<const sqd_program*, int> running_progs;
+    // admission by frontier bytes (opt.frontier_budget): the frontier bytes of
+    // the running jobs (each job's cuboids after its last round, or the estimate
+    // it was admitted with before its first), and per program the most frontier
+    // bytes per initial subdivision that a finished job held (the estimate)
+    uint64_t frontier_in_flight = 0;
+    std::map<const sqd_program*, double> frontier_per_unit;
+    sqd_service_stats stats{};
     bool stop = false;
     std::vector<std::thread> workers;
 
@@ -331,11 +338,44 @@ struct sqd_service {
                     lk.unlock();
                     sqd_core_destroy(c);
                     lk.lock();
+                    ++stats.cores_destroyed;
                     freed = true;
                     break;
                 }
             if(!freed) cv.wait(lk);
         }
+    }
+
+    // Frontier bytes of one cuboid of `prog` (its bounds and its depth).
+    static uint64_t cuboid_bytes(const sqd_program* prog) {
+        return 16ull * static_cast<uint64_t>(std::max(1, sqd_program_nargs(prog))) + 4ull;
+    }
+    // With the lock held: the frontier bytes job `id` is admitted with (at
+    // least one cuboid per initial subdivision of its range).
+    uint64_t frontier_estimate(uint64_t id) {
+        const Job& j = jobs[id];
+        const double range = static_cast<double>(j.last - j.first);
+        double per = static_cast<double>(cuboid_bytes(j.prog));
+        auto f = frontier_per_unit.find(j.prog);
+        if(f != frontier_per_unit.end()) per = std::max(per, f->second);
+        return static_cast<uint64_t>(per * range);
+    }
+    // With the lock held: whether job `id` fits the frontier budget now (a job
+    // is always admitted when no job runs, so that every job ends).
+    bool frontier_fits(uint64_t id) {
+        return opt.frontier_budget == 0 || stats.rows_running == 0 ||
+               frontier_in_flight + frontier_estimate(id) <= opt.frontier_budget;
+    }
+    void frontier_add(int64_t delta) {
+        frontier_in_flight = static_cast<uint64_t>(static_cast<int64_t>(frontier_in_flight) + delta);
+        stats.frontier_bytes = frontier_in_flight;
+        stats.frontier_bytes_peak = std::max(stats.frontier_bytes_peak, frontier_in_flight);
+    }
+    // With the lock held: the oldest queued job of `prog` (0: none), left queued.
+    uint64_t peek_queued(const sqd_program* prog) {
+        for(uint64_t id : queue)
+            if(jobs[id].prog == prog) return id;
+        return 0;
     }
 
     // With the lock held: the oldest queued job of `prog`, taken off the queue (0: none).
@@ -365,6 +405,7 @@ struct sqd_service {
         j.result = r;
         j.error = error;
         j.done = true;
+        ++stats.jobs_completed;
     }
 
     // A worker takes the program of the oldest queued job, a core of that
@@ -388,6 +429,7 @@ struct sqd_service {
                 for(uint64_t id : queue) ++waiting[jobs[id].prog];
                 for(uint64_t id : queue) {
                     const sqd_program* p = jobs[id].prog;
+                    if(!frontier_fits(id)) continue;
                     auto r = running_progs.find(p);
                     if(r == running_progs.end() || r->second == 0) { prog = p; return true; }
                     if(waiting[p] < opt.batch_jobs) continue;
@@ -441,6 +483,7 @@ struct sqd_service {
                 const int rc = sqd_core_create(device, prog, &s, &core);
                 const std::string err = rc != SQD_OK ? sqd_last_error() : std::string();
                 lk.lock();
+                if(rc == SQD_OK) ++stats.cores_created;
                 if(rc != SQD_OK) {                  // creation failed: give the room back, fail one job
                     Slot* sl = find_slot(slot);
                     bytes_in_use -= sl->bytes;
@@ -457,13 +500,17 @@ struct sqd_service {
             const uint32_t rows = opt.batch_jobs;
             std::vector<uint64_t> row_job(rows, 0);
             std::vector<uint32_t> row_first(rows, 0), row_last(rows, 0);   // read without the lock
+            std::vector<uint64_t> row_bytes(rows, 0);                       // frontier bytes counted
+            std::vector<uint64_t> row_cuboids(rows, 0);
+            const uint64_t bpc = cuboid_bytes(prog);
             uint32_t running = 0;
             auto admit = [&] {   // with the lock held: queued jobs of prog into free rows
                 const bool joining = running > 0;
                 for(uint32_t r = 0; r < rows; ++r) {
                     if(row_job[r]) continue;
+                    const uint64_t next = peek_queued(prog);
+                    if(!next || !frontier_fits(next)) break;
                     const uint64_t id = take_queued(prog);
-                    if(!id) break;
                     Job& j = jobs[id];
                     j.started = Clock::now();
                     j.joined = joining;
@@ -474,7 +521,11 @@ struct sqd_service {
                     row_job[r] = id;
                     row_first[r] = j.first;
                     row_last[r] = j.last;
+                    row_bytes[r] = frontier_estimate(id);
+                    frontier_add(static_cast<int64_t>(row_bytes[r]));
                     ++running;
+                    ++stats.rows_running;
+                    stats.rows_peak = std::max(stats.rows_peak, stats.rows_running);
                 }
             };
             admit();
@@ -484,6 +535,7 @@ struct sqd_service {
                 uint64_t finished = 0;
                 uint32_t active = 0;
                 const int rc = sqd_batch_step(core, jc.data(), &finished, &active);
+                if(rc == SQD_OK) sqd_batch_frontier(core, row_cuboids.data(), rows);
                 std::vector<int> status(rows, SQD_OK);
                 std::vector<std::string> errors(rows);
                 if(rc != SQD_OK) {
@@ -501,10 +553,23 @@ struct sqd_service {
                 }
                 lk.lock();
                 for(uint32_t r = 0; r < rows; ++r) {
-                    if(!row_job[r] || !((finished >> r) & 1ull)) continue;
+                    if(!row_job[r]) continue;
+                    if(!((finished >> r) & 1ull)) {          // still running: its frontier now
+                        const uint64_t now = row_cuboids[r] * bpc;
+                        frontier_add(static_cast<int64_t>(now) - static_cast<int64_t>(row_bytes[r]));
+                        row_bytes[r] = now;
+                        continue;
+                    }
+                    if(status[r] == SQD_OK && row_last[r] > row_first[r]) {
+                        double& u = frontier_per_unit[prog];
+                        u = std::max(u, static_cast<double>(jc[r].frontier_peak * bpc) / (row_last[r] - row_first[r]));
+                    }
+                    frontier_add(-static_cast<int64_t>(row_bytes[r]));
+                    row_bytes[r] = 0;
                     complete(row_job[r], jc[r], status[r], errors[r]);
                     row_job[r] = 0;
                     --running;
+                    --stats.rows_running;
                 }
                 cv.notify_all();
                 admit();   // jobs that arrived while the round ran join at this boundary
@@ -624,6 +689,13 @@ int sqd_service_wait(sqd_service* s, uint64_t job, sqd_job_result* out) {
     s->jobs.erase(it);
     if(j.result.status != SQD_OK) sqd::set_last_error(j.error);
     return j.result.status;
+}
+
+int sqd_service_get_stats(sqd_service* s, sqd_service_stats* out) {
+    if(!s || !out) return fail(SQD_ERR_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->stats;
+    return SQD_OK;
 }
 
 void sqd_service_destroy(sqd_service* s) {

@@ -63,7 +63,17 @@ class ServiceOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("workers", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64),
                 ("fp32_prefilter", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("level_bytes", ctypes.c_uint64),
                 ("frontier_capacity", ctypes.c_uint64), ("core_bytes_estimate", ctypes.c_uint64),
-                ("batch_jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("batch_jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("frontier_budget", ctypes.c_uint64)]
+
+
+class ServiceStats(ctypes.Structure):
+    _fields_ = [("frontier_bytes", ctypes.c_uint64), ("frontier_bytes_peak", ctypes.c_uint64),
+                ("rows_running", ctypes.c_uint32), ("rows_peak", ctypes.c_uint32),
+                ("cores_created", ctypes.c_uint64), ("cores_destroyed", ctypes.c_uint64),
+                ("jobs_completed", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 class JobCounts(ctypes.Structure):
@@ -145,6 +155,7 @@ def lib():
             "sqd_batch_admit": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
             "sqd_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(JobCounts), ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_uint32)]),
+            "sqd_batch_frontier": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), _SZ]),
             "sqd_core_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
             "sqd_core_memory": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_autotune_options_default": (None, [ctypes.POINTER(AutotuneOptions)]),
@@ -158,6 +169,7 @@ def lib():
             "sqd_service_set_settings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Settings)]),
             "sqd_service_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
             "sqd_service_destroy": (None, [_P]),
+            "sqd_service_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServiceStats)]),
             "sqd_core_set_profiling": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "sqd_core_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _SZ, ctypes.POINTER(_SZ)]),
             "sqd_rounded_op": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D, _D]),
@@ -189,11 +201,11 @@ def exported_symbols():
             "sqd_program_text", "sqd_program_nargs", "sqd_program_has_kernels", "sqd_lemma_count",
             "sqd_lemma_key", "sqd_lemma_title", "sqd_core_create", "sqd_core_destroy", "sqd_initial_box",
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_prove_range",
-            "sqd_prove_batch", "sqd_core_job_criticals", "sqd_batch_admit", "sqd_batch_step",
+            "sqd_prove_batch", "sqd_core_job_criticals", "sqd_batch_admit", "sqd_batch_step", "sqd_batch_frontier",
             "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
             "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
             "sqd_service_get_settings", "sqd_service_set_settings",
-            "sqd_service_wait", "sqd_service_destroy",
+            "sqd_service_wait", "sqd_service_destroy", "sqd_service_get_stats",
             "sqd_core_set_profiling",
             "sqd_core_kernel_stats", "sqd_core_acos_fallbacks", "sqd_core_fp32_stats", "sqd_width_stats",
             "sqd_core_set_progress", "sqd_core_set_critical_sink", "sqd_prove_range_noalloc", "sqd_eval_program",
@@ -397,6 +409,12 @@ class Core:
         """Continuous batch: the job [first, last) takes the free row `row` and
         starts with the next batch_step."""
         _check(lib().sqd_batch_admit(self._h, row, first, last))
+
+    def batch_frontier(self, rows):
+        """Cuboids on each row's frontier after the last batch step."""
+        arr = (ctypes.c_uint64 * rows)()
+        _check(lib().sqd_batch_frontier(self._h, arr, rows))
+        return [int(v) for v in arr]
 
     def batch_step(self, rows):
         """One round of the continuous batch on a core of `rows` rows: a dict
@@ -606,6 +624,13 @@ class Service:
         r = JobResult()
         _check(lib().sqd_service_wait(self._h, job, ctypes.byref(r)))
         return r.as_dict()
+
+    def stats(self):
+        """Frontier bytes in flight and their peak, running rows and their peak,
+        cores created and destroyed, jobs completed (sqd_service_get_stats)."""
+        st = ServiceStats()
+        _check(lib().sqd_service_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
 
     def close(self):
         if self._h:

@@ -106,3 +106,25 @@ def test_service_admits_jobs_into_a_running_batch():
         assert r["status"] == 0 and r["counts"] == want[rg]["counts"], rg
     assert any(r["joined_running"] for r in res)
     assert all(r["latency_seconds"] < first["latency_seconds"] for r in res if r["joined_running"])
+
+
+def test_service_admission_by_frontier_bytes():
+    # a frontier budget limits the jobs that run together; the counts stay the
+    # counts of each range proved alone
+    prog = sqd.Program.from_lemma("two")
+    ranges = [(16 * i, 16 * i + 16) for i in range(8)]
+    kw = dict(fp32_prefilter=0, dequeue_buffer_size=256)
+    want = _alone(prog, ranges, **kw)
+    peaks = {}
+    for budget in (0, 1):
+        with sqd.Service(workers=2, batch_jobs=8, fp32_prefilter=0, frontier_budget=budget) as svc:
+            svc.set_settings(prog, sqd.default_settings(**kw))
+            ids = [svc.submit(prog, a, b) for a, b in ranges]
+            res = [svc.wait(i) for i in ids]
+            st = svc.stats()
+        for rg, r in zip(ranges, res):
+            assert r["status"] == 0 and r["counts"] == want[rg]["counts"], (budget, rg)
+        assert st["jobs_completed"] == len(ranges) and st["rows_running"] == 0 and st["frontier_bytes"] == 0
+        peaks[budget] = st["rows_peak"]
+    assert peaks[0] > 1        # without a budget the batch takes several jobs at once
+    assert peaks[1] == 1       # a budget of one byte: a job is admitted only when none runs
