@@ -267,6 +267,25 @@ __device__ __forceinline__ bool propagate(Num (&lb)[P::N], Num (&ub)[P::N], int 
     return false;
 }
 
+// The inline first step of propagate<P, true> alone: the initial bounds of a
+// change to `arg`, applied in queue order until the first one that changes the
+// box.  -1: a bound emptied the box (propagate returns true); 0: none changed it
+// and the propagation is over with this box (as in propagate); 1: one changed
+// it, and only propagate itself, from the start, gives the propagated box.
+template<class P, class Num>
+__device__ __forceinline__ int propagate_initial(Num (&lb)[P::N], Num (&ub)[P::N], int arg, const Num* C) {
+    constexpr int NB = P::NB;
+    if(NB == 0) return 0;
+    int m = 0;
+    P::deps(arg * 3 + 2, [&](int) { ++m; });
+    if(m >= NB) return 0;   // a full queue reads as empty: nothing is applied
+    int r_first = 0;
+    P::deps(arg * 3 + 2, [&](int k) {
+        if(r_first == 0) r_first = P::apply(k, lb, ub, C);
+    });
+    return r_first < 0 ? -1 : (r_first > 0 ? 1 : 0);
+}
+
 // ---------------------------------------------------------------- wave helpers
 __device__ __forceinline__ unsigned long long wave_append(bool flag, unsigned long long* counter) {
     const unsigned long long mask = __ballot(flag);
@@ -519,14 +538,19 @@ __device__ __forceinline__ void static_decide(const LevelArgs& args, unsigned lo
 // the rows are checked with the hoisted terms when propagation left every other
 // variable bit for bit as in the parent (the child's box is then, in every
 // variable those terms read, the box h was computed for), and in full otherwise.
+// DEFER: the child is decided here only when the level's initial bounds leave
+// the box unchanged (then the hoisted terms apply) or empty it; otherwise
+// `defer` is set and nothing is counted or written for it: the caller decides
+// it later with static_decide, together with other such children.
 // Called by every lane of the wave.
-template<class P, int MODE, int L>
+template<class P, int MODE, int L, bool DEFER>
 __device__ __forceinline__ void static_decide_hoisted(const LevelArgs& args, bool valid, const double (&plb)[P::N],
                                                       const double (&pub)[P::N], double pl, double pu, double wd, int i,
                                                       unsigned origin, unsigned leaf,
                                                       const typename P::template Hoist<L, double>& h,
                                                       const double* s_c, PropScratch<P>& s_prop,
-                                                      unsigned long long& c_leaf, unsigned long long& c_crit) {
+                                                      unsigned long long& c_leaf, unsigned long long& c_crit,
+                                                      bool& defer) {
     constexpr int N = P::N;
     constexpr int ARG = P::st_arg(L);
     constexpr int SUBS = P::st_subs(L);
@@ -535,16 +559,27 @@ __device__ __forceinline__ void static_decide_hoisted(const LevelArgs& args, boo
     double lb[N], ub[N];
 #pragma unroll
     for(int a = 0; a < N; ++a) { lb[a] = plb[a]; ub[a] = pub[a]; }
+    defer = false;
     if(valid) {
         lb[ARG] = split_point(pl, pu, wd, i, SUBS);
         ub[ARG] = split_point(pl, pu, wd, i + 1, SUBS);
-        if(propagate<P, P::prop_fast(L)>(lb, ub, ARG, N, 4u, s_c, s_prop)) {
+        bool pruned;
+        if constexpr(DEFER) {
+            const int r = propagate_initial<P>(lb, ub, ARG, s_c);
+            pruned = r < 0;
+            defer = r > 0;
+        } else {
+            pruned = propagate<P, P::prop_fast(L)>(lb, ub, ARG, N, 4u, s_c, s_prop);
+        }
+        if(defer) {
+        } else if(pruned) {
             ++c_leaf;
         } else {
+            // with DEFER the box is the parent's apart from the level's variable
             bool same = true;
 #pragma unroll
             for(int a = 0; a < N; ++a)
-                if(a != ARG) same = same && bits(lb[a]) == bits(plb[a]) && bits(ub[a]) == bits(pub[a]);
+                if(!DEFER && a != ARG) same = same && bits(lb[a]) == bits(plb[a]) && bits(ub[a]) == bits(pub[a]);
             const bool violated = same ? P::template rows_violated_hoisted<L>(lb, ub, s_c, h)
                                        : P::template rows_violated<L>(lb, ub, s_c);
             if(violated) {
@@ -629,7 +664,17 @@ void k_static_level(LevelArgs job0) {
         using H = typename P::template Hoist<L, double>;
         __shared__ H s_h[kBlock / 64];
         H& h = s_h[threadIdx.x / 64];
+        // Children whose initial bounds change another variable (a few percent;
+        // they need the full propagation and the full rows) are listed per wave
+        // and decided 64 at a time, so that a wave does not run that path for
+        // one or two of its lanes.  The list holds fewer than 64 children
+        // between parents, so 64 + SUBS entries suffice.
+        constexpr bool DEFER = P::prop_fast(L);
+        __shared__ unsigned long long s_def[kBlock / 64][DEFER ? 64 + SUBS : 1];
+        unsigned long long* const dq = s_def[threadIdx.x / 64];
+        unsigned dcount = 0;   // wave-uniform
         const int lane = static_cast<int>(__lane_id());
+        const unsigned long long below = (1ull << lane) - 1ull;
         const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x / 64);
         const unsigned long long wave = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
         const unsigned long long wtrips = (n_items + waves - 1) / waves;   // the same for every wave
@@ -658,12 +703,32 @@ void k_static_level(LevelArgs job0) {
 #pragma unroll 1
             for(int k = 0; k < SUBS / 64; ++k) {
                 const int i = lane + 64 * k;
-                static_decide_hoisted<P, MODE, L>(args, valid, plb, pub, pl, pu, wd, i, p_origin,
-                                                  p_leaf * SUBS + static_cast<unsigned>(i), h, s_c, s_prop, c_leaf,
-                                                  c_crit);
+                bool defer;
+                static_decide_hoisted<P, MODE, L, DEFER>(args, valid, plb, pub, pl, pu, wd, i, p_origin,
+                                                         p_leaf * SUBS + static_cast<unsigned>(i), h, s_c, s_prop,
+                                                         c_leaf, c_crit, defer);
+                if constexpr(DEFER) {
+                    const unsigned long long m = __ballot(defer);
+                    if(defer) dq[dcount + __popcll(m & below)] = item * SUBS + static_cast<unsigned>(i);
+                    dcount += static_cast<unsigned>(__popcll(m));
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
+            if constexpr(DEFER) {
+                // full trips of 64 while the list holds them; after the last
+                // parent, the rest
+                const bool last = trip + 1 == wtrips;
+                while(dcount >= 64 || (last && dcount > 0)) {
+                    const unsigned take = dcount >= 64 ? 64u : dcount;
+                    dcount -= take;
+                    const bool v = static_cast<unsigned>(lane) < take;
+                    const unsigned long long w = v ? dq[dcount + lane] : 0ull;
+                    static_decide<P, MODE, L>(args, in_base, v, w, s_c, s_prop, c_leaf, c_crit);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     } else {
         const unsigned long long trips = (total + stride - 1) / stride;
