@@ -355,6 +355,14 @@ __device__ __forceinline__ void load_consts_f32(float* dst, const double* src) {
     __syncthreads();
 }
 
+// A value equal on every lane of the wave, held in scalar registers from here
+// on (the compiler cannot tell that a value loaded from memory or derived from
+// threadIdx.x / 64 is the same on all lanes, and keeps it in vector registers).
+__device__ __forceinline__ unsigned long long wave_uniform(unsigned long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
 __device__ __forceinline__ unsigned long long input_base(const LevelArgs& a) { return a.in_base ? *a.in_base : 0ull; }
 // Inputs [base, base + n) of a level, n clamped to the input list's capacity (a
 // count past it was raised by a pass that could not store its survivors, which
@@ -663,7 +671,11 @@ void k_static_level(LevelArgs job0) {
         constexpr int ARG = P::st_arg(L);
         using H = typename P::template Hoist<L, double>;
         __shared__ H s_h[kBlock / 64];
-        H& h = s_h[threadIdx.x / 64];
+        // the wave's loop state is the same on all its lanes: in scalar
+        // registers it leaves the vector registers to the deferred children's
+        // full decision, which spills otherwise
+        const unsigned wib = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);   // wave in block
+        H& h = s_h[wib];
         // Children whose initial bounds change another variable (a few percent;
         // they need the full propagation and the full rows) are listed per wave
         // and decided 64 at a time, so that a wave does not run that path for
@@ -671,23 +683,25 @@ void k_static_level(LevelArgs job0) {
         // between parents, so 64 + SUBS entries suffice.
         constexpr bool DEFER = P::prop_fast(L);
         __shared__ unsigned long long s_def[kBlock / 64][DEFER ? 64 + SUBS : 1];
-        unsigned long long* const dq = s_def[threadIdx.x / 64];
+        unsigned long long* const dq = s_def[wib];
         unsigned dcount = 0;   // wave-uniform
         const int lane = static_cast<int>(__lane_id());
         const unsigned long long below = (1ull << lane) - 1ull;
         const unsigned long long waves = static_cast<unsigned long long>(gridDim.x) * (blockDim.x / 64);
-        const unsigned long long wave = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
-        const unsigned long long wtrips = (n_items + waves - 1) / waves;   // the same for every wave
+        const unsigned long long wave = static_cast<unsigned long long>(blockIdx.x) * (blockDim.x / 64) + wib;
+        const unsigned long long items = wave_uniform(n_items);
+        const unsigned long long base = wave_uniform(in_base);
+        const unsigned long long wtrips = (items + waves - 1) / waves;   // the same for every wave
         for(unsigned long long trip = 0; trip < wtrips; ++trip) {
             const unsigned long long item = wave + trip * waves;
-            const bool valid = item < n_items;                            // wave-uniform
+            const bool valid = item < items;                              // wave-uniform
             double plb[N], pub[N];
             double pl = 0, pu = 0, wd = 0;
             unsigned p_origin = 0, p_leaf = 0;
 #pragma unroll
             for(int a = 0; a < N; ++a) { plb[a] = 0.0; pub[a] = 0.0; }
             if(valid) {
-                const unsigned long long src = in_base + item;
+                const unsigned long long src = base + item;
                 load_box<P>(args.in, src, plb, pub);
                 if(args.in.origin) { p_origin = args.in.origin[src]; p_leaf = args.in.leaf[src]; }
                 else { p_origin = static_cast<unsigned>(item); p_leaf = 0; }
@@ -724,7 +738,7 @@ void k_static_level(LevelArgs job0) {
                     dcount -= take;
                     const bool v = static_cast<unsigned>(lane) < take;
                     const unsigned long long w = v ? dq[dcount + lane] : 0ull;
-                    static_decide<P, MODE, L>(args, in_base, v, w, s_c, s_prop, c_leaf, c_crit);
+                    static_decide<P, MODE, L>(args, base, v, w, s_c, s_prop, c_leaf, c_crit);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
