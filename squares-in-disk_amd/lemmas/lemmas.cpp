@@ -225,6 +225,19 @@ ConstraintSystem expected_criticals_4d() {
                              w + x + y + z >= 0);
 }
 
+// A known-answer system of this implementation's own (no reference
+// counterpart), shaped for the per-parent paths of the static levels: the last
+// static level splits z 128 ways, its rows hold terms free of z (hoisted once
+// per parent: sqrt(x + y)), z <= y lets a child's z range raise y's lower bound
+// (a child that changes another variable is decided apart from its siblings),
+// and the final generation leaves critical boxes.  Its counts and critical
+// boxes are checked against the CPU oracle.
+ConstraintSystem expected_criticals_hoisted() {
+    const Expr x = x0, y = x1, z = x2;
+    return constraint_system(variable(x, "x", 0_Z, 1_Z), variable(y, "y", 0_Z, 1_Z), variable(z, "z", 0_Z, 1_Z),
+                             z <= y, sqrt(x + y) + z >= 1_Z);
+}
+
 constexpr std::uint64_t dyn(std::uint32_t i, std::uint32_t s) { return ivarp::dynamic_subdivision(i, s); }
 
 } // namespace
@@ -252,6 +265,8 @@ const std::vector<LemmaDef>& all_lemmas() {
         // known-answer systems of the reference's tests; totals are for generation_count = 2
         {"ec1", "expected criticals, 2 variables", expected_criticals_2d, {dyn(8, 2), 4}, 408, 352},
         {"ec2", "expected criticals, 4 variables", expected_criticals_4d, {dyn(4, 4), dyn(4, 4), 8, 8}, 614868, 541696},
+        {"ec3", "expected criticals, 3 variables, a 128-way last static level", expected_criticals_hoisted,
+         {dyn(4, 2), 8, 128}, 45436, 45056},
     };
     return defs;
 }

@@ -21,7 +21,7 @@ import numpy as np
 
 import point_eval
 
-KEYS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven", "more1", "more2", "ec1", "ec2"]
+KEYS = ["top1", "top2", "one", "two", "three", "four", "five", "six", "seven", "more1", "more2", "ec1", "ec2", "ec3"]
 SEED = 20261203
 
 

@@ -72,23 +72,25 @@ def test_batch_partial_repeated_and_replayed():
         assert (r["counts"]["num_cuboids"], r["counts"]["num_leaf_cuboids"]) == PUBLISHED["two"]
 
 
-@pytest.mark.parametrize("fp32", [False, True])
-def test_batch_criticals_match_oracle(program_text, fp32):
-    # the 4-variable statement with expected criticals: each job's critical boxes
-    # are the oracle's for its range, bit for bit
-    p = sqd.Program.from_lemma("ec2")
+@pytest.mark.parametrize("key,fp32", [("ec2", False), ("ec2", True), ("ec3", False)])
+def test_batch_criticals_match_oracle(program_text, key, fp32):
+    # the systems with expected criticals (ec3: its last static level decided per
+    # parent, children that change another variable apart): each job's critical
+    # boxes are the oracle's for its range, bit for bit
+    p = sqd.Program.from_lemma(key)
     s = sqd.default_settings(generation_count=1, jobs=4, fp32_prefilter=sqd.FP32_ALL_LEVELS if fp32 else 0)
     with sqd.Core(p, settings=s) as core:
         n, _ = core.shape()
         ranges = [(0, n), (0, max(1, n // 3)), (n // 3, n), (n // 2, n // 2 + 1)]
         got = core.prove_batch(ranges, critical_capacity=1 << 19)
     for (a, b), g in zip(ranges, got):
-        o = pyoracle.Oracle(program_text("ec2"), fp32=fp32)
+        o = pyoracle.Oracle(program_text(key), fp32=fp32)
         o.set_init_range(a, b)
         oc, ocrit = o.run(generations=1, threads=8, max_criticals=1 << 19)
         assert g["counts"] == oc, (a, b)
         assert g["n_critical"] == oc["num_critical_cuboids"] == len(ocrit)
-        gb = np.vstack([g["crit_lb"], g["crit_ub"]]).T[:, [0, 4, 1, 5, 2, 6, 3, 7]]
+        nv = g["crit_lb"].shape[0]
+        gb = np.vstack([g["crit_lb"], g["crit_ub"]]).T[:, [k for a_ in range(nv) for k in (a_, nv + a_)]]
         ob = ocrit.reshape(len(ocrit), -1)
         gs = gb[np.lexsort(gb.T[::-1])]
         os_ = ob[np.lexsort(ob.T[::-1])]

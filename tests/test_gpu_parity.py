@@ -128,7 +128,8 @@ def test_initial_queue(cores, program_text, key):
 
 
 @pytest.mark.parametrize("key,limit", [("one", 128), ("two", 512), ("three", 256), ("four", 128), ("five", 256),
-                                       ("six", 128), ("seven", 64), ("more1", 256), ("more2", 64), ("ec2", 16)])
+                                       ("six", 128), ("seven", 64), ("more1", 256), ("more2", 64), ("ec2", 16),
+                                       ("ec3", 4)])
 def test_nonfinal_hulls(cores, program_text, key, limit):
     o = pyoracle.Oracle(program_text(key))
     qlb, qub, _ = o.initial_queue()
@@ -201,3 +202,21 @@ def test_prove_expected_criticals_4d(cores, program_text, gens):
     gs = g[np.lexsort(g.T[::-1])]
     os_ = o[np.lexsort(o.T[::-1])]
     assert np.array_equal(gs.view(np.uint64), os_.view(np.uint64))
+
+
+@pytest.mark.parametrize("gens", [0, 1, 2])
+def test_prove_expected_criticals_hoisted(cores, program_text, gens):
+    # ec3: the last static level splits z 128 ways and is decided per parent (its
+    # terms free of z hoisted), the children whose z range raises y's lower bound
+    # apart from their siblings; counts and every critical box are the oracle's,
+    # on the first proof of the core and on a replay of its round graphs
+    o_counts, o_crit = pyoracle.Oracle(program_text("ec3")).run(generations=gens, threads=8, max_criticals=1 << 16)
+    o = o_crit.reshape(len(o_crit), -1)
+    os_ = o[np.lexsort(o.T[::-1])]
+    core = cores("ec3", generations=gens)
+    for _ in range(2):
+        r = core.prove(critical_capacity=1 << 16)
+        assert r["counts"] == o_counts and r["n_critical"] == len(o_crit) > 0
+        g = np.vstack([r["crit_lb"], r["crit_ub"]]).T[:, [0, 3, 1, 4, 2, 5]]
+        gs = g[np.lexsort(g.T[::-1])]
+        assert np.array_equal(gs.view(np.uint64), os_.view(np.uint64))

@@ -145,7 +145,7 @@ def test_program_text_rejects_garbage():
 
 def test_every_lemma_has_kernels():
     have = {k: sqd.Program.from_lemma(k).has_kernels for k, _ in sqd.lemmas()}
-    assert all(have.values()) and len(have) == 13
+    assert all(have.values()) and len(have) == 14
 
 
 def test_c_abi_exports_every_declared_function():
