@@ -272,8 +272,9 @@ __host__ __device__ inline u128 cos_fx_fast(u128 b) {
     for(int k = kCosTermsFast - 2; k >= 0; --k) s = T.c[k] - mulq_low(z, s);
     return s;
 }
-// cos_ge at the cheaper precision; -1 also when that does not decide.
-__host__ __device__ inline int cos_ge_fast(double y, double x) {
+// cos_ge at the cheaper precision; -1 also when that does not decide.  *d
+// receives cos(y) - x at scale 2^124 (within kCosErrFast units).
+__host__ __device__ inline int cos_ge_fast(double y, double x, i128* d = nullptr) {
     const u128 Y = static_cast<u128>(to_fx_signed(y));
     const u128 H = half_pi_fx();
     const bool neg = Y >= H;
@@ -283,6 +284,7 @@ __host__ __device__ inline int cos_ge_fast(double y, double x) {
     const i128 c = neg ? -static_cast<i128>(s) : static_cast<i128>(s);
     const i128 X = to_fx_signed(x);
     const i128 e = static_cast<i128>(kCosErrFast);
+    if(d) *d = c - X;
     if(c - e > X) return 1;
     if(c + e < X) return 0;
     return -1;
@@ -298,12 +300,42 @@ inline int cos_ge_full(double y, double x) { return cos_ge(y, x); }
 // cos_ge, tried at the cheaper precision first where that usually decides
 // (y at least 2^-10 away from 0 and pi: neighbouring cosines at least about
 // 2^-63 apart).
-__host__ __device__ inline int cos_ge_2(double y, double x) {
+// *d (when the cheaper test decided; *fast tells) as in cos_ge_fast.
+__host__ __device__ inline int cos_ge_2(double y, double x, i128* d = nullptr, bool* fast = nullptr) {
     if(y >= 0x1p-10 && y <= 0x1.91ffb54442d18p+1) {       // pi - 2^-10, rounded down
-        const int t = cos_ge_fast(y, x);
-        if(t >= 0) return t;
+        const int t = cos_ge_fast(y, x, d);
+        if(t >= 0) {
+            if(fast) *fast = true;
+            return t;
+        }
     }
+    if(fast) *fast = false;
     return cos_ge_full(y, x);
+}
+
+// The walk's second step from its first one's value, for y0 in [2^-10, pi -
+// 2^-10] where the first step's cheaper test gives d = cos(y0) - x within
+// 2^-86.  The neighbour's value follows from the slope:
+//   cos(y0) - cos(y0 + u) = 2 sin(y0 + u/2) sin(u/2) = u sin(y0) (1 + eps)
+// for the spacing u of the doubles at y0 (u <= 2^-51), |eps| < 2^-40 since
+// sin(y0) >= sin(2^-10) > 2^-11; the same for y0 - u'.  With |d| <= 2^-45,
+// sin(y0) = sqrt(1 - cos^2 y0) is sqrt((1 - x)(1 + x)) within a relative 2^-22
+// (|cos^2 y0 - x^2| <= 2^-44 against sin^2 y0 > 2^-22), plus the few roundings
+// of the double evaluation, in any rounding mode.  Each side is widened by a
+// relative 2^-20 (and d by 2^-80 absolute).  True when the walk ends at the
+// neighbour:
+//   up (cos(y0) >= x, t = 1): cos(y0 + u) < x, i.e. d < u sin(y0) (1 + eps);
+//   down (cos(y0) < x, t = 0): cos(y0 - u') >= x, i.e. -d <= u' sin(y0) (1 + eps).
+// False inside the band and whenever the walk goes on past the neighbour.
+__host__ __device__ inline bool acos_slope_ends(double x, double y0, double y1, i128 d) {
+    const u128 m = static_cast<u128>(d < 0 ? -d : d);
+    if(m >> 79) return false;                                        // |d| >= 2^-45
+    const double md = (static_cast<double>(static_cast<std::uint64_t>(m >> 32)) * 0x1p32 +
+                       static_cast<double>(static_cast<std::uint64_t>(m & 0xffffffffull))) * 0x1p-124;
+    const double m_hi = md * (1.0 + 0x1p-20) + 0x1p-80;              // |d|, rounded up generously
+    const double sn = __builtin_sqrt((1.0 - x) * (1.0 + x));        // sin(y0) within a relative 2^-22
+    const double u = y1 > y0 ? y1 - y0 : y0 - y1;                    // exact: a power of two
+    return m_hi < u * sn * (1.0 - 0x1p-20);
 }
 
 // RD(acos(x)) for x in (-1, 1) by walking from an approximation y0 with cos_ge;
@@ -320,9 +352,21 @@ __host__ __device__ inline bool acos_walk(double x, double y0, double* lo) {
     int dir = 0;                                      // +1 up (cos(y0) >= x), -1 down
     double prev = y;
     for(int i = 0; i < 9; ++i) {
-        const int t = cos_ge_2(y, x);
+        i128 d = 0;
+        bool fast = false;
+        const int t = cos_ge_2(y, x, &d, &fast);
         if(t < 0) return false;
-        if(i == 0) dir = t == 1 ? 1 : -1;
+        if(i == 0) {
+            dir = t == 1 ? 1 : -1;
+            // the second step from the slope, when it decides (acos_slope_ends)
+            if(fast) {
+                const double y1 = from_bits_h(to_bits_h(y) + (t == 1 ? 1 : -1));
+                if(acos_slope_ends(x, y, y1, d)) {
+                    *lo = t == 1 ? y : y1;
+                    return true;
+                }
+            }
+        }
         if(dir > 0) {
             if(t == 0) { *lo = prev; return true; }
             if(y == pi_down) { *lo = y; return true; }
