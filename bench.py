@@ -115,7 +115,7 @@ def parse():
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_static_level_one_subcontainer.txt"),
                     help="HBM traffic summary (FETCH_SIZE/WRITE_SIZE per dispatch) to attach to the roofline")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight in the batched legs")
-    ap.add_argument("--batch-jobs", type=int, default=16, help="proofs per batch (one grid row each)")
+    ap.add_argument("--batch-jobs", type=int, default=32, help="proofs per batch (one grid row each)")
     ap.add_argument("--stream-batch-jobs", type=int, default=4,
                     help="jobs per batch in the job-stream leg (11 programs share the memory budget)")
     ap.add_argument("--batched-proofs", type=int, default=4096)
@@ -275,14 +275,11 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
         core.set_profiling(False)
     with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
         svc.set_settings(prog, best)                          # the settings reported above
-        for _ in range(2):                                    # creates and warms every worker's core
-            # deep enough that every worker starts a batch (a program gets a
-            # further batch while a whole batch of its jobs waits)
-            warm = [svc.submit(prog, 0, n) for _ in range(4 * workers * batch_jobs)]
-            for i in warm:
-                svc.wait(i)
+        warm_service(svc, prog, n, workers, batch_jobs)
+        # all proofs handed over in one call, so that the timed region measures
+        # the service, not the rate of one host thread's submissions
         t0 = time.perf_counter()
-        ids = [svc.submit(prog, 0, n) for _ in range(proofs)]
+        ids = svc.submit_many(prog, [(0, n)] * proofs)
         res = [svc.wait(i) for i in ids]
         elapsed = time.perf_counter() - t0
     for r in res:
@@ -307,6 +304,21 @@ def fp32_first_batched(sqd, prog, expected, workers, batch_jobs, proofs, plain_v
     return out
 
 
+def warm_service(svc, prog, n, workers, batch_jobs):
+    """Bursts of whole proofs until the service stops making cores (at least two
+    bursts, at most eight): a burst deep enough that every worker starts a batch
+    (a program gets a further batch while whole batches of its jobs wait), so
+    that no core is made, allocated and captured inside the timed region."""
+    made = -1
+    for k in range(8):
+        for i in svc.submit_many(prog, [(0, n)] * (4 * workers * batch_jobs)):
+            svc.wait(i)
+        now = svc.stats()["cores_created"]
+        if k >= 1 and now == made:
+            break
+        made = now
+
+
 def service_batches(sqd, prog, settings, expected, workers, batch_jobs, proofs):
     """`proofs` whole proofs of `prog` through a service whose cores use `settings`
     (`workers` workers, `batch_jobs` proofs per batch); every proof's counts are
@@ -314,14 +326,11 @@ def service_batches(sqd, prog, settings, expected, workers, batch_jobs, proofs):
     n = initial_subdivisions(prog)
     with sqd.Service(workers=workers, batch_jobs=batch_jobs) as svc:
         svc.set_settings(prog, settings)
-        for _ in range(2):                                    # creates and warms every worker's core
-            # deep enough that every worker starts a batch (a program gets a
-            # further batch while a whole batch of its jobs waits)
-            warm = [svc.submit(prog, 0, n) for _ in range(4 * workers * batch_jobs)]
-            for i in warm:
-                svc.wait(i)
+        warm_service(svc, prog, n, workers, batch_jobs)
+        # all proofs handed over in one call, so that the timed region measures
+        # the service, not the rate of one host thread's submissions
         t0 = time.perf_counter()
-        ids = [svc.submit(prog, 0, n) for _ in range(proofs)]
+        ids = svc.submit_many(prog, [(0, n)] * proofs)
         res = [svc.wait(i) for i in ids]
         elapsed = time.perf_counter() - t0
     for r in res:

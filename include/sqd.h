@@ -376,6 +376,14 @@ typedef struct sqd_job_result {
 void sqd_service_options_default(sqd_service_options* options);
 int sqd_service_create(const sqd_service_options* options, sqd_service** out);
 int sqd_service_submit(sqd_service* service, const sqd_program* program, uint32_t first, uint32_t last, uint64_t* job);
+/* n jobs of `program` at once, job i over [first[i], last[i]); their ids in
+ * jobs[0..n).  One lock and one wake-up of the workers for all of them, for
+ * callers that hand over many jobs together (a job stream's burst, a whole
+ * proof's ranges).  No counterpart in the reference (which proves one
+ * statement at a time, prover/proof_driver.hpp:393-438).  All ranges are
+ * checked before any job is queued: on an error none is. */
+int sqd_service_submit_many(sqd_service* service, const sqd_program* program, const uint32_t* first,
+                            const uint32_t* last, size_t n, uint64_t* jobs);
 /* The settings the service's cores of `program` use (*known = 0 before its first
  * job has chosen them), with jobs = batch_jobs. */
 int sqd_service_get_settings(sqd_service* service, const sqd_program* program, int* known, sqd_settings* out);

@@ -667,6 +667,28 @@ int sqd_service_submit(sqd_service* s, const sqd_program* prog, uint32_t first, 
     return SQD_OK;
 }
 
+int sqd_service_submit_many(sqd_service* s, const sqd_program* prog, const uint32_t* first, const uint32_t* last,
+                            size_t n, uint64_t* jobs) {
+    if(!s || !prog || (n && (!first || !last || !jobs))) return fail(SQD_ERR_ARGUMENT, "null argument");
+    for(size_t i = 0; i < n; ++i)
+        if(first[i] >= last[i]) return fail(SQD_ERR_ARGUMENT, "empty initial range (job " + std::to_string(i) + ")");
+    if(n == 0) return SQD_OK;
+    std::lock_guard<std::mutex> lk(s->m);
+    const Clock::time_point now = Clock::now();
+    for(size_t i = 0; i < n; ++i) {
+        const uint64_t id = s->next_id++;
+        sqd_service::Job& j = s->jobs[id];
+        j.prog = prog;
+        j.first = first[i];
+        j.last = last[i];
+        j.submitted = now;
+        s->queue.push_back(id);
+        jobs[i] = id;
+    }
+    s->cv.notify_all();
+    return SQD_OK;
+}
+
 int sqd_service_poll(sqd_service* s, uint64_t job, int* done, sqd_job_result* out) {
     if(!s || !done) return fail(SQD_ERR_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(s->m);

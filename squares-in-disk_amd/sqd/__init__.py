@@ -164,6 +164,8 @@ def lib():
             "sqd_service_options_default": (None, [ctypes.POINTER(ServiceOptions)]),
             "sqd_service_create": (ctypes.c_int, [ctypes.POINTER(ServiceOptions), ctypes.POINTER(_P)]),
             "sqd_service_submit": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+            "sqd_service_submit_many": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                                       ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
             "sqd_service_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(JobResult)]),
             "sqd_service_get_settings": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Settings)]),
             "sqd_service_set_settings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Settings)]),
@@ -203,7 +205,8 @@ def exported_symbols():
             "sqd_initial_queue", "sqd_nonfinal", "sqd_final", "sqd_split", "sqd_prove", "sqd_prove_range",
             "sqd_prove_batch", "sqd_core_job_criticals", "sqd_batch_admit", "sqd_batch_step", "sqd_batch_frontier",
             "sqd_core_criticals", "sqd_core_shape", "sqd_core_memory", "sqd_autotune_options_default", "sqd_autotune",
-            "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_poll",
+            "sqd_service_options_default", "sqd_service_create", "sqd_service_submit", "sqd_service_submit_many",
+            "sqd_service_poll",
             "sqd_service_get_settings", "sqd_service_set_settings",
             "sqd_service_wait", "sqd_service_destroy", "sqd_service_get_stats",
             "sqd_core_set_profiling",
@@ -613,6 +616,15 @@ class Service:
         j = ctypes.c_uint64()
         _check(lib().sqd_service_submit(self._h, program._h, first, last, ctypes.byref(j)))
         return int(j.value)
+
+    def submit_many(self, program, ranges):
+        """Jobs over the (first, last) ranges, queued together; their ids."""
+        n = len(ranges)
+        first = (ctypes.c_uint32 * max(n, 1))(*[a for a, _ in ranges])
+        last = (ctypes.c_uint32 * max(n, 1))(*[b for _, b in ranges])
+        ids = (ctypes.c_uint64 * max(n, 1))()
+        _check(lib().sqd_service_submit_many(self._h, program._h, first, last, n, ids))
+        return [int(ids[i]) for i in range(n)]
 
     def poll(self, job):
         done = ctypes.c_int()

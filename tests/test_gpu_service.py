@@ -161,3 +161,23 @@ def test_service_fixed_settings_and_batches():
         with pytest.raises(sqd.SqdError):
             svc.set_settings(p, s)                           # the program has cores now
     assert [r["counts"] for r in res] == want
+
+
+def test_service_submit_many():
+    # jobs handed over in one call: each gets its range's counts, and a call
+    # with an empty range is refused before any of its jobs is queued
+    p = sqd.Program.from_lemma("two")
+    ranges = [(0, 128), (0, 64), (64, 128), (5, 6), (100, 128), (1, 127), (0, 128), (31, 97)]
+    with sqd.Core(p) as core:
+        want = [core.prove_range(a, b)["counts"] for a, b in ranges]
+    with sqd.Service(workers=2, batch_jobs=4) as svc:
+        with pytest.raises(sqd.SqdError):
+            svc.submit_many(p, [(0, 8), (9, 9), (10, 20)])
+        assert svc.submit_many(p, []) == []
+        ids = svc.submit_many(p, ranges)
+        assert len(set(ids)) == len(ranges)
+        res = [svc.wait(i) for i in ids]
+        st = svc.stats()
+    assert [r["status"] for r in res] == [0] * len(ranges)
+    assert [r["counts"] for r in res] == want
+    assert st["jobs_completed"] == len(ranges)          # nothing of the refused call ran
