@@ -109,13 +109,15 @@ def test_fp32_prove_verdicts(fp32_cores, key):
     assert decided <= checked
 
 
+@pytest.mark.parametrize("key", ["ec2", "ec3"])
 @pytest.mark.parametrize("gens", [0, 1, 2])
-def test_fp32_expected_criticals_4d(fp32_cores, program_text, gens):
-    r = fp32_cores("ec2", generations=gens).prove(critical_capacity=1 << 19)
-    o = pyoracle.Oracle(program_text("ec2"), fp32=True)
+def test_fp32_expected_criticals(fp32_cores, program_text, key, gens):
+    r = fp32_cores(key, generations=gens).prove(critical_capacity=1 << 19)
+    o = pyoracle.Oracle(program_text(key), fp32=True)
     o_counts, o_crit = o.run(generations=gens, threads=8, max_criticals=1 << 19)
     assert r["counts"] == o_counts
-    g = np.vstack([r["crit_lb"], r["crit_ub"]]).T[:, [0, 4, 1, 5, 2, 6, 3, 7]]
+    nv = r["crit_lb"].shape[0]
+    g = np.vstack([r["crit_lb"], r["crit_ub"]]).T[:, [k for a in range(nv) for k in (a, nv + a)]]
     oo = o_crit.reshape(len(o_crit), -1)
     gs = g[np.lexsort(g.T[::-1])]
     os_ = oo[np.lexsort(oo.T[::-1])]

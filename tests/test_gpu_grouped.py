@@ -48,16 +48,18 @@ def test_grouped_nonfinal_hulls_bit_exact(program_text, key, limit, group):
     assert np.array_equal(_bits(ghu[:, keep]), _bits(ohu[:, keep]))
 
 
+@pytest.mark.parametrize("key", ["ec2", "ec3"])
 @pytest.mark.parametrize("gens", [0, 1, 2])
-def test_grouped_criticals_bit_exact(program_text, gens):
-    p = sqd.Program.from_lemma("ec2")
+def test_grouped_criticals_bit_exact(program_text, key, gens):
+    p = sqd.Program.from_lemma(key)
     s = sqd.default_settings(generation_count=gens, group_levels=ALL, prefilter_group=8)
     with sqd.Core(p, settings=s) as core:
         r = core.prove(critical_capacity=1 << 19)
-    o = pyoracle.Oracle(program_text("ec2"))
+    o = pyoracle.Oracle(program_text(key))
     oc, ocrit = o.run(generations=gens, threads=8, max_criticals=1 << 19)
     assert r["counts"] == oc
-    g = np.vstack([r["crit_lb"], r["crit_ub"]]).T[:, [0, 4, 1, 5, 2, 6, 3, 7]]
+    nv = r["crit_lb"].shape[0]
+    g = np.vstack([r["crit_lb"], r["crit_ub"]]).T[:, [k for a in range(nv) for k in (a, nv + a)]]
     oo = ocrit.reshape(len(ocrit), -1)
     assert np.array_equal(g[np.lexsort(g.T[::-1])].view(np.uint64), oo[np.lexsort(oo.T[::-1])].view(np.uint64))
 
